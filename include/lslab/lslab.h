@@ -1,0 +1,78 @@
+/**
+ * @file
+ * Common pieces of the header-only C++ API over the lslab C ABI.
+ *
+ * The reference reports nothing: host errors are ignored or exit() the
+ * process (include/lslab/lslab.h:28-41 of the reference).  Here every C ABI
+ * status other than LSLAB_OK becomes an lslab::error exception.
+ */
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+
+#include "../lslab_c.h"
+
+namespace lslab {
+
+struct error : std::runtime_error {
+    int code;
+    explicit error(int c, const char* what = "")
+        : std::runtime_error(std::string("lslab: ") + lslab_error_string(c) +
+                             (what && *what ? std::string(" in ") + what : std::string())),
+          code(c) {}
+};
+
+inline void check(int rc, const char* what = "") {
+    if (rc != LSLAB_OK) throw error(rc, what);
+}
+
+/// Identity hash: the reference's hash<T> (include/lslab/hash.h:10-15).  The
+/// device side always buckets by (size_t)key & (2^n - 1); this type exists so
+/// that code naming lslab::hash<K> keeps compiling.
+template <typename T>
+struct hash {
+    size_t operator()(T x) const { return static_cast<size_t>(x); }
+};
+
+/// Layout of cuda::std::pair<bool, V>: the reference's get() result
+/// (include/lslab/map.h:208-211).
+template <typename V>
+struct found_value {
+    bool first;
+    V second;
+};
+
+/// Layout of cuda::std::pair<K, V>: the reference's AoS put() input
+/// (include/lslab/map.h:189-192).
+template <typename K, typename V>
+struct key_value {
+    K first;
+    V second;
+};
+
+namespace detail {
+template <typename T>
+constexpr bool slab_word() {
+    return std::is_integral<T>::value && (sizeof(T) == 4 || sizeof(T) == 8);
+}
+
+/// The reference chooses the CTA size per call as a template argument
+/// (map.h:189-220).  Wave64 kernels here take 64..256 threads per block.
+template <int block_size>
+inline void apply_block_size(lslab_table* t) {
+    static_assert(block_size >= 32 && block_size <= 1024, "block_size");
+    constexpr int bt = block_size < 64 ? 64 : (block_size > 256 ? 256 : block_size);
+    lslab_tuning tu;
+    check(lslab_get_tuning(t, &tu), "get_tuning");
+    if (tu.block_threads != bt) {
+        tu.block_threads = bt;
+        check(lslab_set_tuning(t, &tu), "set_tuning");
+    }
+}
+}  // namespace detail
+
+}  // namespace lslab

@@ -1,0 +1,92 @@
+/**
+ * @file
+ * lslab::set<K> -- header-only host API over the lslab C ABI.
+ *
+ * Mirrors the reference's set (include/lslab/set.h:21-123 of the reference):
+ * contains / insert / remove batches over device pointers, bool results.
+ * insert reports true for present and newly inserted keys alike.  The
+ * thrust::device_vector overloads become a (pointer, size) pair.
+ */
+#pragma once
+
+#include "lslab.h"
+
+namespace lslab {
+
+template <typename K, typename Hash = hash<K>>
+class set {
+    static_assert(detail::slab_word<K>(), "K must be a 4- or 8-byte integer");
+
+public:
+    set() : set(10) {}
+
+    explicit set(unsigned n_log_2, uint64_t pool_slabs = 10000,
+                 const lslab_tuning* tuning = nullptr)
+        : n_log_2_(n_log_2) {
+        check(lslab_set_create(&t_, sizeof(K), n_log_2, pool_slabs, tuning), "set_create");
+    }
+
+    set(const set&) = delete;
+    set& operator=(const set&) = delete;
+    set(set&& o) noexcept : t_(o.t_), n_log_2_(o.n_log_2_) { o.t_ = nullptr; }
+    set& operator=(set&& o) noexcept {
+        if (this != &o) {
+            if (t_) lslab_destroy(t_);
+            t_ = o.t_;
+            n_log_2_ = o.n_log_2_;
+            o.t_ = nullptr;
+        }
+        return *this;
+    }
+    ~set() {
+        if (t_) lslab_destroy(t_);
+    }
+
+    template <int block_size = 256>
+    void contains(const K* keys, bool* output, size_t size, hipStream_t stream = 0) {
+        detail::apply_block_size<block_size>(t_);
+        check(lslab_set_contains(t_, keys, reinterpret_cast<uint8_t*>(output), size, stream),
+              "set::contains");
+    }
+
+    template <int block_size = 256>
+    void insert(const K* keys, bool* output, size_t size, hipStream_t stream = 0) {
+        detail::apply_block_size<block_size>(t_);
+        check(lslab_set_insert(t_, keys, reinterpret_cast<uint8_t*>(output), size, stream),
+              "set::insert");
+    }
+
+    template <int block_size = 256>
+    void remove(const K* keys, bool* output, size_t size, hipStream_t stream = 0) {
+        detail::apply_block_size<block_size>(t_);
+        check(lslab_set_remove(t_, keys, reinterpret_cast<uint8_t*>(output), size, stream),
+              "set::remove");
+    }
+
+    void mixed(const uint8_t* ops, const K* keys, bool* output, size_t size,
+               hipStream_t stream = 0) {
+        check(lslab_set_mixed(t_, ops, keys, reinterpret_cast<uint8_t*>(output), size, stream),
+              "set::mixed");
+    }
+
+    void exec_batches(size_t nbatches, const uint64_t* offsets, const uint8_t* ops, const K* keys,
+                      bool* output, hipStream_t stream = 0) {
+        check(lslab_exec_batches(t_, nbatches, offsets, ops, keys, nullptr, output, stream),
+              "set::exec_batches");
+    }
+
+    unsigned buckets() const { return 1u << n_log_2_; }
+    int status(hipStream_t stream = 0) { return lslab_status(t_, stream); }
+    lslab_stats stats(hipStream_t stream = 0) {
+        lslab_stats s;
+        check(lslab_stats_get(t_, &s, stream), "set::stats");
+        return s;
+    }
+    lslab_table* handle() { return t_; }
+
+private:
+    lslab_table* t_ = nullptr;
+    unsigned n_log_2_;
+};
+
+}  // namespace lslab

@@ -1,0 +1,153 @@
+/*
+ * lslab C ABI -- MI355X (gfx950) slab-list hash map and set.
+ *
+ * The drop-in boundary of the LSlab batch API.  The reference exposes a
+ * header-only CUDA C++ template API (include/lslab/map.h:24-25,
+ * include/lslab/set.h:21-22) whose host batch methods launch one kernel per
+ * batch on a caller stream.  This ABI carries the same calls as plain C
+ * functions over an opaque table handle, with key/value widths chosen at
+ * creation (4 or 8 bytes each), caller-owned DEVICE pointers for every input
+ * and output, a hipStream_t per call, and an int status instead of the
+ * reference's silent failures and device traps.  The header-only C++ classes
+ * in include/lslab/map.h and include/lslab/set.h sit on top of it.
+ *
+ * Threading: every call is asynchronous on its stream.  Calls on one table
+ * are ordered by the library (each call waits for the table's previous call,
+ * whatever stream it used), which is the linearisation the reference obtains
+ * from its per-bucket locks (include/lslab/warp_mutex.h:13-59).
+ *
+ * Device-side failures (slab pool exhausted, a lock or barrier spin that hit
+ * its time bound) are sticky per table and reported by lslab_status().
+ */
+#ifndef LSLAB_C_H
+#define LSLAB_C_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  The reference has none: allocation errors are ignored
+ * (map.h:45-51), pool exhaustion traps (detail/device_allocator.h:65-68). */
+#define LSLAB_OK 0
+#define LSLAB_ERR_INVALID 1        /* bad argument or unsupported width */
+#define LSLAB_ERR_HIP 2            /* a HIP runtime call failed */
+#define LSLAB_ERR_OOM 3            /* device allocation failed */
+#define LSLAB_ERR_POOL_EXHAUSTED 4 /* an insert needed a slab beyond the pool */
+#define LSLAB_ERR_TIMEOUT 5        /* a device spin exceeded its time bound */
+#define LSLAB_ERR_CORRUPT 6        /* a chain link or batch index failed its bounds check */
+
+/* Opcodes of the mixed-op entry points.  Maps: get/put/remove; sets:
+ * contains/insert/remove.  Same numbering as the legacy per-request types of
+ * the reference (test/slabunified_test.cu.old:60-67), minus EMPTY. */
+#define LSLAB_OP_GET 0
+#define LSLAB_OP_PUT 1
+#define LSLAB_OP_REMOVE 2
+
+/* Engines.  PLAIN: the reference protocol, one wave64 group per op holding
+ * its bucket's reader/writer lock for the traversal.  TWO_PHASE: lock-free
+ * reads of the pre-batch table plus one atomic exchange per mutation on a
+ * batch-local per-key chain, then one write-back per mutated key. */
+#define LSLAB_ENGINE_PLAIN 0
+#define LSLAB_ENGINE_TWO_PHASE 1
+
+typedef struct lslab_table lslab_table;
+
+typedef struct {
+    int lanes_per_op;      /* lanes probing one slab per op: 4, 8, 16, 32 (0 = 16) */
+    int engine;            /* LSLAB_ENGINE_* for put/remove/mixed calls (0 = PLAIN) */
+    int block_threads;     /* threads per block (0 = 256) */
+    int max_blocks;        /* grid cap for the per-batch kernels (0 = auto) */
+    uint64_t max_subbatch; /* two-phase ops per sub-batch; sizes the combining
+                              table and per-op scratch (0 = 2^20) */
+} lslab_tuning;
+
+typedef struct {
+    uint64_t buckets;      /* 2^n_log2 */
+    uint64_t pool_slabs;   /* pool capacity */
+    uint64_t slabs_used;   /* pool slabs handed out (min(bump pointer, pool)) */
+    uint64_t present;      /* keys whose valid bit is set */
+    uint64_t max_chain;    /* longest bucket chain, in slabs */
+    uint64_t slab_bytes;   /* bytes per slab */
+    uint64_t device_bytes; /* bytes of device memory the table holds */
+} lslab_stats;
+
+/* Creation / destruction.  Reference: map(n_log_2) / set(n_log_2)
+ * (map.h:43-52, set.h:31-40) with device_allocator(stack_size = 10000)
+ * (device_allocator.h:16, detail/device_allocator.h:9-14).  Unlike the
+ * reference the destructor frees everything.  n_log2 <= 30 as in the
+ * reference's int shift (map.h:44).  tuning may be NULL. */
+int lslab_map_create(lslab_table** out, int key_bytes, int value_bytes, unsigned n_log2,
+                     uint64_t pool_slabs, const lslab_tuning* tuning);
+int lslab_set_create(lslab_table** out, int key_bytes, unsigned n_log2, uint64_t pool_slabs,
+                     const lslab_tuning* tuning);
+int lslab_destroy(lslab_table* t);
+int lslab_set_tuning(lslab_table* t, const lslab_tuning* tuning);
+int lslab_get_tuning(const lslab_table* t, lslab_tuning* out);
+
+/* Map batch calls.
+ * put_pairs: map::put(pair<K,V>* ops, V* out, n, stream)   map.h:189-192
+ *            (pairs laid out as cuda::std::pair<K,V>: K, then V at alignof)
+ * put:       map::put(K* keys, V* vals, V* out, n, stream)  map.h:198-201
+ *            out[i] = previous value, 0 if the key was absent (map.h:136-152)
+ * get:       map::get(K* keys, pair<bool,V>* out, n, stream) map.h:208-211
+ *            out[i] = {found, value}; value 0 when absent
+ * remove:    new; the reference traversal supports REMOVE on map nodes
+ *            (detail/traverse.h:88-91) but map.h does not expose it.
+ *            out[i] = {was present, removed value}
+ * mixed:     new; per-op opcodes (LSLAB_OP_*), out[i] = {found, value} with the
+ *            meaning of the op's single-type call (put: {was present, old}). */
+int lslab_map_put_pairs(lslab_table* t, const void* pairs, void* out_old, size_t n,
+                        hipStream_t stream);
+int lslab_map_put(lslab_table* t, const void* keys, const void* values, void* out_old, size_t n,
+                  hipStream_t stream);
+int lslab_map_get(lslab_table* t, const void* keys, void* out_pairs, size_t n,
+                  hipStream_t stream);
+int lslab_map_remove(lslab_table* t, const void* keys, void* out_pairs, size_t n,
+                     hipStream_t stream);
+int lslab_map_mixed(lslab_table* t, const uint8_t* ops, const void* keys, const void* values,
+                    void* out_pairs, size_t n, hipStream_t stream);
+
+/* Set batch calls: set::insert / contains / remove(const K*, bool*, n, stream)
+ * (set.h:92-123).  insert reports true for present and newly inserted keys
+ * alike (detail/traverse.h:204-212,226-232); remove reports whether the key
+ * was present.  mixed: per-op opcodes (GET = contains, PUT = insert). */
+int lslab_set_insert(lslab_table* t, const void* keys, uint8_t* out, size_t n,
+                     hipStream_t stream);
+int lslab_set_contains(lslab_table* t, const void* keys, uint8_t* out, size_t n,
+                       hipStream_t stream);
+int lslab_set_remove(lslab_table* t, const void* keys, uint8_t* out, size_t n,
+                     hipStream_t stream);
+int lslab_set_mixed(lslab_table* t, const uint8_t* ops, const void* keys, uint8_t* out, size_t n,
+                    hipStream_t stream);
+
+/* Batched form (new): nbatches mixed-op batches laid end to end in ops/keys/
+ * values/out, batch b = [offsets[b], offsets[b+1]) with offsets a DEVICE
+ * array of nbatches+1 uint64.  One persistent launch runs them in order with
+ * the two-phase engine (batch b sees every effect of batches < b), so the
+ * call is one graph node when captured.  out as lslab_map_mixed (maps) or
+ * bool per op (sets, values ignored). */
+int lslab_exec_batches(lslab_table* t, size_t nbatches, const uint64_t* offsets,
+                       const uint8_t* ops, const void* keys, const void* values, void* out,
+                       hipStream_t stream);
+
+/* Synchronises the stream, returns the table's sticky device status
+ * (LSLAB_OK, LSLAB_ERR_POOL_EXHAUSTED, LSLAB_ERR_TIMEOUT) and clears it. */
+int lslab_status(lslab_table* t, hipStream_t stream);
+
+/* Introspection (synchronous).  dump writes up to cap present pairs into the
+ * DEVICE arrays keys_out / vals_out (vals_out ignored for sets; order
+ * unspecified) and the number of present keys into *count. */
+int lslab_stats_get(lslab_table* t, lslab_stats* out, hipStream_t stream);
+int lslab_dump(lslab_table* t, void* keys_out, void* vals_out, uint64_t cap, uint64_t* count,
+               hipStream_t stream);
+
+const char* lslab_error_string(int status);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LSLAB_C_H */

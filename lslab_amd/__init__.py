@@ -1,0 +1,24 @@
+"""lslab_amd: MI355X-native LSlab slab-list hash map and set.
+
+The product is liblslab.so (hand-written gfx950 HIP kernels behind the C ABI
+in include/lslab_c.h) and the header-only C++ API in include/lslab/.  This
+Python package binds the same C ABI for tests and benchmarks.
+"""
+from . import _lib  # noqa: F401
+from ._lib import (ENGINE_PLAIN, ENGINE_TWO_PHASE, OP_GET, OP_PUT, OP_REMOVE,  # noqa: F401
+                   LslabError)
+
+__all__ = ["Map", "Set", "tuning", "workload", "ENGINE_PLAIN", "ENGINE_TWO_PHASE", "OP_GET",
+           "OP_PUT", "OP_REMOVE", "LslabError"]
+
+
+def __getattr__(name):
+    # torch-dependent parts load lazily so that the C ABI can be inspected
+    # without initialising a device.
+    if name in ("Map", "Set", "tuning"):
+        from . import table
+        return getattr(table, name)
+    if name == "workload":
+        from . import workload
+        return workload
+    raise AttributeError(name)

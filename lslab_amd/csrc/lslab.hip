@@ -1,0 +1,680 @@
+// lslab: HIP kernels for gfx950 and the C ABI declared in include/lslab_c.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "../../include/lslab_c.h"
+#include "engine.hpp"
+
+using namespace lslab_dev;
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+// Grid-wide barrier state for the persistent batched kernel.
+struct BarState {
+    uint32_t count;
+    uint32_t gen;
+    uint32_t abort;
+    uint32_t pad;
+};
+
+// Plain per-batch kernel: one G-lane group per op, grid-stride.
+template <class L, int G, bool LOCKED>
+__global__ __launch_bounds__(256) void k_plain(TableDev t, OpsDev o, OutDev out, uint64_t n) {
+    using K = typename L::K;
+    using V = typename L::V;
+    const uint64_t ng = (uint64_t)gridDim.x * blockDim.x / G;
+    const uint64_t gid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+    for (uint64_t i = gid; i < n; i += ng) {
+        uint8_t op;
+        K key;
+        V val;
+        load_op<L>(o, i, op, key, val);
+        bool rf, ok;
+        V rv;
+        run_op<L, G, LOCKED>(t, op, key, val, rf, rv, ok);
+        if (Grp<G>::lane() == 0) write_result<L>(out, i, rf, rv);
+    }
+}
+
+template <class L, int G>
+__global__ __launch_bounds__(256) void k_phase_r(TableDev t, OpsDev o, OutDev out, uint64_t lo,
+                                                 uint64_t m) {
+    const uint64_t ng = (uint64_t)gridDim.x * blockDim.x / G;
+    const uint64_t gid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+    for (uint64_t li = gid; li < m; li += ng) phase_r_op<L, G>(t, o, out, lo, (uint32_t)li);
+}
+
+template <class L, int G>
+__global__ __launch_bounds__(256) void k_phase_w(TableDev t, OpsDev o, uint64_t lo, uint64_t m) {
+    const uint64_t ng = (uint64_t)gridDim.x * blockDim.x / G;
+    const uint64_t gid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+    for (uint64_t li = gid; li < m; li += ng) phase_w_op<L, G>(t, o, lo, (uint32_t)li);
+}
+
+// Grid barrier for a grid sized to full residency.  Every wave publishes its
+// writes (agent release), one lane per block arrives, the last arrival bumps
+// the generation, everyone acquires.  Spins are bounded; on timeout or abort
+// the barrier returns false and the caller leaves the kernel.
+__device__ __forceinline__ bool grid_sync(BarState* bar, uint32_t nblocks) {
+    release_agent();
+    __syncthreads();
+    __shared__ uint32_t ok_s;
+    if (threadIdx.x == 0) {
+        uint32_t ok = 1;
+        const uint32_t g = ld_agent(&bar->gen);
+        const uint32_t arrived =
+            __hip_atomic_fetch_add(&bar->count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (arrived == nblocks - 1u) {
+            __hip_atomic_store(&bar->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&bar->gen, g + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            const uint64_t t0 = now_ticks();
+            while (ld_agent(&bar->gen) == g) {
+                if (ld_agent(&bar->abort) || now_ticks() - t0 > SPIN_LIMIT_TICKS) {
+                    __hip_atomic_store(&bar->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ok = 0;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        ok_s = ok;
+    }
+    __syncthreads();
+    acquire_agent();
+    return ok_s != 0;
+}
+
+// Persistent batched kernel: batches in order, each as two-phase sub-batches
+// separated by grid barriers.
+template <class L, int G>
+__global__ __launch_bounds__(256) void k_batches(TableDev t, OpsDev o, OutDev out,
+                                                 const uint64_t* offsets, uint64_t nbatches,
+                                                 uint64_t submax, BarState* bar) {
+    const uint64_t ng = (uint64_t)gridDim.x * blockDim.x / G;
+    const uint64_t gid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+    for (uint64_t b = 0; b < nbatches; ++b) {
+        const uint64_t lo = offsets[b], hi = offsets[b + 1];
+        for (uint64_t slo = lo; slo < hi; slo += submax) {
+            const uint64_t m = min(submax, hi - slo);
+            for (uint64_t li = gid; li < m; li += ng) phase_r_op<L, G>(t, o, out, slo, (uint32_t)li);
+            if (!grid_sync(bar, gridDim.x)) {
+                if (threadIdx.x == 0)
+                    __hip_atomic_fetch_or(t.status, ST_TIMEOUT, __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+                return;
+            }
+            for (uint64_t li = gid; li < m; li += ng) phase_w_op<L, G>(t, o, slo, (uint32_t)li);
+            if (!grid_sync(bar, gridDim.x)) {
+                if (threadIdx.x == 0)
+                    __hip_atomic_fetch_or(t.status, ST_TIMEOUT, __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+                return;
+            }
+        }
+    }
+}
+
+__global__ void k_reset_bar(BarState* bar) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        bar->count = 0;
+        bar->abort = 0;
+    }
+}
+
+// Introspection: one thread per bucket walks its chain.
+struct StatsDev {
+    unsigned long long present;
+    unsigned long long max_chain;
+};
+
+template <class L>
+__global__ void k_stats(TableDev t, StatsDev* st) {
+    const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long present = 0, len = 0;
+    if (b < t.nbuckets) {
+        uint64_t s = b;
+        for (;;) {
+            present += __builtin_popcount(*valid_word<L>(t, s) & L::CAPMASK);
+            len++;
+            uint32_t nx = *next_word<L>(t, s);
+            if (!nx) break;
+            if (nx < t.nbuckets || nx >= t.nbuckets + t.pool || len > t.pool + 1) {
+                __hip_atomic_fetch_or(t.status, ST_CORRUPT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            s = nx;
+        }
+    }
+    // wave reduction then one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) {
+        present += __shfl_down(present, off, 64);
+        len = max(len, (unsigned long long)__shfl_down(len, off, 64));
+    }
+    if (__lane_id() == 0) {
+        atomicAdd(&st->present, present);
+        atomicMax(&st->max_chain, len);
+    }
+}
+
+template <class L>
+__global__ void k_dump(TableDev t, typename L::K* keys, typename L::V* vals, uint64_t cap,
+                       unsigned long long* counter) {
+    const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= t.nbuckets) return;
+    uint64_t s = b, hops = 0;
+    for (;;) {
+        const uint32_t v = *valid_word<L>(t, s) & L::CAPMASK;
+        for (int i = 0; i < L::CAP; ++i) {
+            if ((v >> i) & 1u) {
+                unsigned long long idx = atomicAdd(counter, 1ull);
+                if (idx < cap) {
+                    keys[idx] = *key_slot<L>(t, s, i);
+                    if constexpr (L::IS_MAP) {
+                        if (vals) vals[idx] = *val_slot<L>(t, s, i);
+                    }
+                }
+            }
+        }
+        const uint32_t nx = *next_word<L>(t, s);
+        if (!nx) break;
+        if (nx < t.nbuckets || nx >= t.nbuckets + t.pool || ++hops > t.pool) {
+            __hip_atomic_fetch_or(t.status, ST_CORRUPT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+        }
+        s = nx;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+struct DevState {
+    uint64_t alloc_ctr;
+    uint32_t status;
+    uint32_t pad;
+    BarState bar;
+    StatsDev stats;
+    unsigned long long dump_counter;
+};
+
+enum TableType { T_MAP_32_32, T_MAP_32_64, T_MAP_64_32, T_MAP_64_64, T_SET_32, T_SET_64 };
+
+struct lslab_table {
+    TableType type;
+    bool is_map;
+    int key_bytes, val_bytes;
+    unsigned n_log2;
+    uint64_t nbuckets, pool, slab_bytes, device_bytes;
+    lslab_tuning tun;
+    TableDev dev;
+    DevState* state;
+    uint64_t gt_cap, submax;
+    hipEvent_t order_ev;
+    int device;
+    int num_cus;
+};
+
+#define HIPCHK(x)                                   \
+    do {                                            \
+        hipError_t e_ = (x);                        \
+        if (e_ != hipSuccess) return LSLAB_ERR_HIP; \
+    } while (0)
+
+using L_M3232 = Layout<uint32_t, uint32_t, true>;
+using L_M3264 = Layout<uint32_t, uint64_t, true>;
+using L_M6432 = Layout<uint64_t, uint32_t, true>;
+using L_M6464 = Layout<uint64_t, uint64_t, true>;
+using L_S32 = Layout<uint32_t, uint32_t, false>;
+using L_S64 = Layout<uint64_t, uint32_t, false>;
+
+template <typename F>
+static int dispatch(const lslab_table* t, F&& f) {
+    switch (t->type) {
+        case T_MAP_32_32: return f(L_M3232{});
+        case T_MAP_32_64: return f(L_M3264{});
+        case T_MAP_64_32: return f(L_M6432{});
+        case T_MAP_64_64: return f(L_M6464{});
+        case T_SET_32: return f(L_S32{});
+        case T_SET_64: return f(L_S64{});
+    }
+    return LSLAB_ERR_INVALID;
+}
+
+template <typename F>
+static int dispatch_g(int g, F&& f) {
+    switch (g) {
+        case 4: return f(std::integral_constant<int, 4>{});
+        case 8: return f(std::integral_constant<int, 8>{});
+        case 16: return f(std::integral_constant<int, 16>{});
+        case 32: return f(std::integral_constant<int, 32>{});
+    }
+    return LSLAB_ERR_INVALID;
+}
+
+static void normalise(lslab_tuning& tu) {
+    if (tu.lanes_per_op == 0) tu.lanes_per_op = 16;
+    if (tu.block_threads == 0) tu.block_threads = 256;
+    if (tu.max_subbatch == 0) tu.max_subbatch = 1ull << 20;
+}
+
+static bool tuning_ok(const lslab_tuning& tu) {
+    const int g = tu.lanes_per_op;
+    if (g != 4 && g != 8 && g != 16 && g != 32) return false;
+    if (tu.engine != LSLAB_ENGINE_PLAIN && tu.engine != LSLAB_ENGINE_TWO_PHASE) return false;
+    if (tu.block_threads < 64 || tu.block_threads > 256 || (tu.block_threads % 64)) return false;
+    if (tu.max_blocks < 0) return false;
+    if (tu.max_subbatch < 64 || tu.max_subbatch > (1ull << 30)) return false;
+    return true;
+}
+
+static uint64_t grid_for(const lslab_table* t, uint64_t n) {
+    const uint64_t groups_per_block = (uint64_t)t->tun.block_threads / t->tun.lanes_per_op;
+    uint64_t blocks = (n + groups_per_block - 1) / groups_per_block;
+    if (blocks == 0) blocks = 1;
+    if (t->tun.max_blocks > 0) blocks = std::min<uint64_t>(blocks, (uint64_t)t->tun.max_blocks);
+    return std::min<uint64_t>(blocks, 1ull << 30);
+}
+
+static void free_table(lslab_table* t) {
+    if (!t) return;
+    (void)hipFree(t->dev.slabs);
+    (void)hipFree(t->dev.locks);
+    (void)hipFree(t->state);
+    (void)hipFree(t->dev.gt_key);
+    (void)hipFree(t->dev.gt_head);
+    (void)hipFree(t->dev.op_gidx);
+    (void)hipFree(t->dev.op_loc);
+    if (t->order_ev) (void)hipEventDestroy(t->order_ev);
+    delete t;
+}
+
+static int alloc_scratch(lslab_table* t) {
+    uint64_t cap = 1;
+    while (cap < 2 * t->tun.max_subbatch) cap <<= 1;
+    (void)hipFree(t->dev.gt_key);
+    (void)hipFree(t->dev.gt_head);
+    (void)hipFree(t->dev.op_gidx);
+    (void)hipFree(t->dev.op_loc);
+    t->dev.gt_key = nullptr;
+    t->dev.gt_head = nullptr;
+    t->dev.op_gidx = nullptr;
+    t->dev.op_loc = nullptr;
+    if (hipMalloc(&t->dev.gt_key, (cap + 1) * sizeof(uint64_t)) != hipSuccess ||
+        hipMalloc(&t->dev.gt_head, (cap + 1) * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&t->dev.op_gidx, t->tun.max_subbatch * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&t->dev.op_loc, t->tun.max_subbatch * sizeof(uint64_t)) != hipSuccess)
+        return LSLAB_ERR_OOM;
+    HIPCHK(hipMemset(t->dev.gt_key, 0xff, (cap + 1) * sizeof(uint64_t)));
+    HIPCHK(hipMemset(t->dev.gt_head, 0, (cap + 1) * sizeof(uint32_t)));
+    t->gt_cap = cap;
+    t->submax = t->tun.max_subbatch;
+    t->dev.gt_mask = (uint32_t)(cap - 1);
+    return LSLAB_OK;
+}
+
+static int create_common(lslab_table** out, TableType type, bool is_map, int kb, int vb,
+                         unsigned n_log2, uint64_t pool, const lslab_tuning* tuning) {
+    if (!out) return LSLAB_ERR_INVALID;
+    *out = nullptr;
+    if (n_log2 > 30) return LSLAB_ERR_INVALID;
+    lslab_tuning tu{};
+    if (tuning) tu = *tuning;
+    normalise(tu);
+    if (!tuning_ok(tu)) return LSLAB_ERR_INVALID;
+    lslab_table* t = new (std::nothrow) lslab_table();
+    if (!t) return LSLAB_ERR_OOM;
+    t->type = type;
+    t->is_map = is_map;
+    t->key_bytes = kb;
+    t->val_bytes = is_map ? vb : 0;
+    t->n_log2 = n_log2;
+    t->nbuckets = 1ull << n_log2;
+    t->pool = pool;
+    t->tun = tu;
+    dispatch(t, [&](auto l) {
+        t->slab_bytes = decltype(l)::SLAB_BYTES;
+        return 0;
+    });
+    const uint64_t nslabs = t->nbuckets + pool;
+    if (nslabs >= (1ull << 32)) {
+        delete t;
+        return LSLAB_ERR_INVALID;
+    }
+    if (hipGetDevice(&t->device) != hipSuccess ||
+        hipDeviceGetAttribute(&t->num_cus, hipDeviceAttributeMultiprocessorCount, t->device) !=
+            hipSuccess) {
+        delete t;
+        return LSLAB_ERR_HIP;
+    }
+    int rc = LSLAB_OK;
+    const uint64_t slab_total = nslabs * t->slab_bytes;
+    if (hipMalloc(&t->dev.slabs, slab_total) != hipSuccess ||
+        hipMalloc(&t->dev.locks, t->nbuckets * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&t->state, sizeof(DevState)) != hipSuccess) {
+        free_table(t);
+        return LSLAB_ERR_OOM;
+    }
+    if (hipMemset(t->dev.slabs, 0, slab_total) != hipSuccess ||
+        hipMemset(t->dev.locks, 0, t->nbuckets * sizeof(uint32_t)) != hipSuccess ||
+        hipMemset(t->state, 0, sizeof(DevState)) != hipSuccess) {
+        free_table(t);
+        return LSLAB_ERR_HIP;
+    }
+    rc = alloc_scratch(t);
+    if (rc != LSLAB_OK) {
+        free_table(t);
+        return rc;
+    }
+    if (hipEventCreateWithFlags(&t->order_ev, hipEventDisableTiming) != hipSuccess ||
+        hipDeviceSynchronize() != hipSuccess) {
+        free_table(t);
+        return LSLAB_ERR_HIP;
+    }
+    t->dev.alloc_ctr = &t->state->alloc_ctr;
+    t->dev.status = &t->state->status;
+    t->dev.nbuckets = t->nbuckets;
+    t->dev.pool = pool;
+    t->dev.mask = (uint32_t)(t->nbuckets - 1);
+    t->device_bytes = slab_total + t->nbuckets * 4 + sizeof(DevState) + (t->gt_cap + 1) * 12 +
+                      t->submax * 12;
+    *out = t;
+    return LSLAB_OK;
+}
+
+extern "C" int lslab_map_create(lslab_table** out, int key_bytes, int value_bytes, unsigned n_log2,
+                                uint64_t pool_slabs, const lslab_tuning* tuning) {
+    TableType ty;
+    if (key_bytes == 4 && value_bytes == 4) ty = T_MAP_32_32;
+    else if (key_bytes == 4 && value_bytes == 8) ty = T_MAP_32_64;
+    else if (key_bytes == 8 && value_bytes == 4) ty = T_MAP_64_32;
+    else if (key_bytes == 8 && value_bytes == 8) ty = T_MAP_64_64;
+    else return LSLAB_ERR_INVALID;
+    return create_common(out, ty, true, key_bytes, value_bytes, n_log2, pool_slabs, tuning);
+}
+
+extern "C" int lslab_set_create(lslab_table** out, int key_bytes, unsigned n_log2,
+                                uint64_t pool_slabs, const lslab_tuning* tuning) {
+    TableType ty;
+    if (key_bytes == 4) ty = T_SET_32;
+    else if (key_bytes == 8) ty = T_SET_64;
+    else return LSLAB_ERR_INVALID;
+    return create_common(out, ty, false, key_bytes, 0, n_log2, pool_slabs, tuning);
+}
+
+extern "C" int lslab_destroy(lslab_table* t) {
+    if (!t) return LSLAB_ERR_INVALID;
+    (void)hipDeviceSynchronize();
+    free_table(t);
+    return LSLAB_OK;
+}
+
+extern "C" int lslab_set_tuning(lslab_table* t, const lslab_tuning* tuning) {
+    if (!t || !tuning) return LSLAB_ERR_INVALID;
+    lslab_tuning tu = *tuning;
+    normalise(tu);
+    if (!tuning_ok(tu)) return LSLAB_ERR_INVALID;
+    const bool resize = tu.max_subbatch != t->tun.max_subbatch;
+    t->tun = tu;
+    if (resize) {
+        // The scratch may still be in use by queued calls of this table.
+        HIPCHK(hipDeviceSynchronize());
+        return alloc_scratch(t);
+    }
+    return LSLAB_OK;
+}
+
+extern "C" int lslab_get_tuning(const lslab_table* t, lslab_tuning* out) {
+    if (!t || !out) return LSLAB_ERR_INVALID;
+    *out = t->tun;
+    return LSLAB_OK;
+}
+
+// Orders this call after the table's previous call, whatever its stream.
+static int begin_call(lslab_table* t, hipStream_t s) {
+    HIPCHK(hipStreamWaitEvent(s, t->order_ev, 0));
+    return LSLAB_OK;
+}
+static int end_call(lslab_table* t, hipStream_t s) {
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(t->order_ev, s));
+    return LSLAB_OK;
+}
+
+// Runs n ops through the plain kernel (locked or lock-free) or the two-phase
+// kernels, depending on `engine`.
+static int run_ops(lslab_table* t, const OpsDev& o, const OutDev& out, uint64_t n, int engine,
+                   bool read_only, hipStream_t s) {
+    if (n == 0) return LSLAB_OK;
+    int rc = begin_call(t, s);
+    if (rc) return rc;
+    const int bt = t->tun.block_threads;
+    rc = dispatch(t, [&](auto l) {
+        using L = decltype(l);
+        return dispatch_g(t->tun.lanes_per_op, [&](auto gc) {
+            constexpr int G = decltype(gc)::value;
+            if (read_only || engine == LSLAB_ENGINE_PLAIN) {
+                const uint64_t grid = grid_for(t, n);
+                if (read_only)
+                    hipLaunchKernelGGL((k_plain<L, G, false>), dim3((unsigned)grid), dim3(bt), 0, s,
+                                       t->dev, o, out, n);
+                else
+                    hipLaunchKernelGGL((k_plain<L, G, true>), dim3((unsigned)grid), dim3(bt), 0, s,
+                                       t->dev, o, out, n);
+            } else {
+                for (uint64_t lo = 0; lo < n; lo += t->submax) {
+                    const uint64_t m = std::min<uint64_t>(t->submax, n - lo);
+                    const uint64_t grid = grid_for(t, m);
+                    hipLaunchKernelGGL((k_phase_r<L, G>), dim3((unsigned)grid), dim3(bt), 0, s,
+                                       t->dev, o, out, lo, m);
+                    hipLaunchKernelGGL((k_phase_w<L, G>), dim3((unsigned)grid), dim3(bt), 0, s,
+                                       t->dev, o, lo, m);
+                }
+            }
+            return hipGetLastError() == hipSuccess ? LSLAB_OK : LSLAB_ERR_HIP;
+        });
+    });
+    if (rc) return rc;
+    return end_call(t, s);
+}
+
+static OpsDev make_ops(const uint8_t* ops, uint8_t uop, const void* keys, uint32_t kstride,
+                       const void* vals, uint32_t vstride) {
+    OpsDev o;
+    o.ops = ops;
+    o.uop = uop;
+    o.keys = static_cast<const char*>(keys);
+    o.kstride = kstride;
+    o.vals = static_cast<const char*>(vals);
+    o.vstride = vstride;
+    return o;
+}
+
+static OutDev make_out(void* pairs, void* vals, uint8_t* flags) {
+    OutDev d;
+    d.pairs = static_cast<char*>(pairs);
+    d.vals = static_cast<char*>(vals);
+    d.flags = flags;
+    return d;
+}
+
+extern "C" int lslab_map_put_pairs(lslab_table* t, const void* pairs, void* out_old, size_t n,
+                                   hipStream_t stream) {
+    if (!t || !t->is_map || (n && (!pairs || !out_old))) return LSLAB_ERR_INVALID;
+    // cuda::std::pair<K,V>: first at 0, second at alignof(pair) = max(sizeof K, sizeof V).
+    const uint32_t al = (uint32_t)std::max(t->key_bytes, t->val_bytes);
+    const uint32_t stride = 2 * al;
+    OpsDev o = make_ops(nullptr, OP_PUT, pairs, stride, static_cast<const char*>(pairs) + al, stride);
+    return run_ops(t, o, make_out(nullptr, out_old, nullptr), n, t->tun.engine, false, stream);
+}
+
+extern "C" int lslab_map_put(lslab_table* t, const void* keys, const void* values, void* out_old,
+                             size_t n, hipStream_t stream) {
+    if (!t || !t->is_map || (n && (!keys || !values || !out_old))) return LSLAB_ERR_INVALID;
+    OpsDev o = make_ops(nullptr, OP_PUT, keys, t->key_bytes, values, t->val_bytes);
+    return run_ops(t, o, make_out(nullptr, out_old, nullptr), n, t->tun.engine, false, stream);
+}
+
+extern "C" int lslab_map_get(lslab_table* t, const void* keys, void* out_pairs, size_t n,
+                             hipStream_t stream) {
+    if (!t || !t->is_map || (n && (!keys || !out_pairs))) return LSLAB_ERR_INVALID;
+    OpsDev o = make_ops(nullptr, OP_GET, keys, t->key_bytes, nullptr, 0);
+    return run_ops(t, o, make_out(out_pairs, nullptr, nullptr), n, t->tun.engine, true, stream);
+}
+
+extern "C" int lslab_map_remove(lslab_table* t, const void* keys, void* out_pairs, size_t n,
+                                hipStream_t stream) {
+    if (!t || !t->is_map || (n && (!keys || !out_pairs))) return LSLAB_ERR_INVALID;
+    OpsDev o = make_ops(nullptr, OP_REMOVE, keys, t->key_bytes, nullptr, 0);
+    return run_ops(t, o, make_out(out_pairs, nullptr, nullptr), n, t->tun.engine, false, stream);
+}
+
+extern "C" int lslab_map_mixed(lslab_table* t, const uint8_t* ops, const void* keys,
+                               const void* values, void* out_pairs, size_t n, hipStream_t stream) {
+    if (!t || !t->is_map || (n && (!ops || !keys || !values || !out_pairs))) return LSLAB_ERR_INVALID;
+    OpsDev o = make_ops(ops, 0, keys, t->key_bytes, values, t->val_bytes);
+    return run_ops(t, o, make_out(out_pairs, nullptr, nullptr), n, t->tun.engine, false, stream);
+}
+
+extern "C" int lslab_set_insert(lslab_table* t, const void* keys, uint8_t* out, size_t n,
+                                hipStream_t stream) {
+    if (!t || t->is_map || (n && (!keys || !out))) return LSLAB_ERR_INVALID;
+    OpsDev o = make_ops(nullptr, OP_PUT, keys, t->key_bytes, nullptr, 0);
+    return run_ops(t, o, make_out(nullptr, nullptr, out), n, t->tun.engine, false, stream);
+}
+
+extern "C" int lslab_set_contains(lslab_table* t, const void* keys, uint8_t* out, size_t n,
+                                  hipStream_t stream) {
+    if (!t || t->is_map || (n && (!keys || !out))) return LSLAB_ERR_INVALID;
+    OpsDev o = make_ops(nullptr, OP_GET, keys, t->key_bytes, nullptr, 0);
+    return run_ops(t, o, make_out(nullptr, nullptr, out), n, t->tun.engine, true, stream);
+}
+
+extern "C" int lslab_set_remove(lslab_table* t, const void* keys, uint8_t* out, size_t n,
+                                hipStream_t stream) {
+    if (!t || t->is_map || (n && (!keys || !out))) return LSLAB_ERR_INVALID;
+    OpsDev o = make_ops(nullptr, OP_REMOVE, keys, t->key_bytes, nullptr, 0);
+    return run_ops(t, o, make_out(nullptr, nullptr, out), n, t->tun.engine, false, stream);
+}
+
+extern "C" int lslab_set_mixed(lslab_table* t, const uint8_t* ops, const void* keys, uint8_t* out,
+                               size_t n, hipStream_t stream) {
+    if (!t || t->is_map || (n && (!ops || !keys || !out))) return LSLAB_ERR_INVALID;
+    OpsDev o = make_ops(ops, 0, keys, t->key_bytes, nullptr, 0);
+    return run_ops(t, o, make_out(nullptr, nullptr, out), n, t->tun.engine, false, stream);
+}
+
+extern "C" int lslab_exec_batches(lslab_table* t, size_t nbatches, const uint64_t* offsets,
+                                  const uint8_t* ops, const void* keys, const void* values,
+                                  void* out, hipStream_t stream) {
+    if (!t || (nbatches && (!offsets || !ops || !keys || !out))) return LSLAB_ERR_INVALID;
+    if (t->is_map && nbatches && !values) return LSLAB_ERR_INVALID;
+    if (nbatches == 0) return LSLAB_OK;
+    int rc = begin_call(t, stream);
+    if (rc) return rc;
+    OpsDev o = make_ops(ops, 0, keys, t->key_bytes, t->is_map ? values : nullptr, t->val_bytes);
+    OutDev d = t->is_map ? make_out(out, nullptr, nullptr)
+                         : make_out(nullptr, nullptr, static_cast<uint8_t*>(out));
+    BarState* bar = &t->state->bar;
+    const int bt = t->tun.block_threads;
+    rc = dispatch(t, [&](auto l) {
+        using L = decltype(l);
+        return dispatch_g(t->tun.lanes_per_op, [&](auto gc) {
+            constexpr int G = decltype(gc)::value;
+            int per_cu = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batches<L, G>, bt, 0) !=
+                    hipSuccess ||
+                per_cu <= 0)
+                return LSLAB_ERR_HIP;
+            uint64_t grid = (uint64_t)per_cu * (uint64_t)t->num_cus;
+            if (t->tun.max_blocks > 0) grid = std::min<uint64_t>(grid, (uint64_t)t->tun.max_blocks);
+            hipLaunchKernelGGL(k_reset_bar, dim3(1), dim3(64), 0, stream, bar);
+            hipLaunchKernelGGL((k_batches<L, G>), dim3((unsigned)grid), dim3(bt), 0, stream, t->dev,
+                               o, d, offsets, (uint64_t)nbatches, t->submax, bar);
+            return hipGetLastError() == hipSuccess ? LSLAB_OK : LSLAB_ERR_HIP;
+        });
+    });
+    if (rc) return rc;
+    return end_call(t, stream);
+}
+
+extern "C" int lslab_status(lslab_table* t, hipStream_t stream) {
+    if (!t) return LSLAB_ERR_INVALID;
+    HIPCHK(hipStreamWaitEvent(stream, t->order_ev, 0));
+    HIPCHK(hipStreamSynchronize(stream));
+    uint32_t st = 0;
+    HIPCHK(hipMemcpy(&st, &t->state->status, sizeof(st), hipMemcpyDeviceToHost));
+    if (st) HIPCHK(hipMemset(&t->state->status, 0, sizeof(uint32_t)));
+    if (st & ST_CORRUPT) return LSLAB_ERR_CORRUPT;
+    if (st & ST_TIMEOUT) return LSLAB_ERR_TIMEOUT;
+    if (st & ST_POOL_EXHAUSTED) return LSLAB_ERR_POOL_EXHAUSTED;
+    return LSLAB_OK;
+}
+
+extern "C" int lslab_stats_get(lslab_table* t, lslab_stats* out, hipStream_t stream) {
+    if (!t || !out) return LSLAB_ERR_INVALID;
+    HIPCHK(hipStreamWaitEvent(stream, t->order_ev, 0));
+    HIPCHK(hipMemsetAsync(&t->state->stats, 0, sizeof(StatsDev), stream));
+    const uint64_t grid = (t->nbuckets + 255) / 256;
+    int rc = dispatch(t, [&](auto l) {
+        using L = decltype(l);
+        hipLaunchKernelGGL(k_stats<L>, dim3((unsigned)grid), dim3(256), 0, stream, t->dev,
+                           &t->state->stats);
+        return hipGetLastError() == hipSuccess ? LSLAB_OK : LSLAB_ERR_HIP;
+    });
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(stream));
+    DevState hs;
+    HIPCHK(hipMemcpy(&hs, t->state, sizeof(DevState), hipMemcpyDeviceToHost));
+    out->buckets = t->nbuckets;
+    out->pool_slabs = t->pool;
+    out->slabs_used = std::min<uint64_t>(hs.alloc_ctr, t->pool);
+    out->present = hs.stats.present;
+    out->max_chain = hs.stats.max_chain;
+    out->slab_bytes = t->slab_bytes;
+    out->device_bytes = t->device_bytes;
+    HIPCHK(hipEventRecord(t->order_ev, stream));
+    return LSLAB_OK;
+}
+
+extern "C" int lslab_dump(lslab_table* t, void* keys_out, void* vals_out, uint64_t cap,
+                          uint64_t* count, hipStream_t stream) {
+    if (!t || !count || (cap && !keys_out)) return LSLAB_ERR_INVALID;
+    HIPCHK(hipStreamWaitEvent(stream, t->order_ev, 0));
+    HIPCHK(hipMemsetAsync(&t->state->dump_counter, 0, sizeof(unsigned long long), stream));
+    const uint64_t grid = (t->nbuckets + 255) / 256;
+    int rc = dispatch(t, [&](auto l) {
+        using L = decltype(l);
+        hipLaunchKernelGGL(k_dump<L>, dim3((unsigned)grid), dim3(256), 0, stream, t->dev,
+                           static_cast<typename L::K*>(keys_out),
+                           static_cast<typename L::V*>(vals_out), cap, &t->state->dump_counter);
+        return hipGetLastError() == hipSuccess ? LSLAB_OK : LSLAB_ERR_HIP;
+    });
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(stream));
+    unsigned long long c = 0;
+    HIPCHK(hipMemcpy(&c, &t->state->dump_counter, sizeof(c), hipMemcpyDeviceToHost));
+    *count = c;
+    HIPCHK(hipEventRecord(t->order_ev, stream));
+    return LSLAB_OK;
+}
+
+extern "C" const char* lslab_error_string(int status) {
+    switch (status) {
+        case LSLAB_OK: return "ok";
+        case LSLAB_ERR_INVALID: return "invalid argument";
+        case LSLAB_ERR_HIP: return "HIP runtime error";
+        case LSLAB_ERR_OOM: return "device out of memory";
+        case LSLAB_ERR_POOL_EXHAUSTED: return "slab pool exhausted";
+        case LSLAB_ERR_TIMEOUT: return "device spin time bound exceeded";
+        case LSLAB_ERR_CORRUPT: return "table structure check failed";
+    }
+    return "unknown status";
+}

@@ -1,0 +1,290 @@
+// Slab layout and wave64 device building blocks for the LSlab map and set on
+// gfx950.
+//
+// Slab layout (one slab = one key block, then one value block for maps):
+//   key block: 32 slots of K.  Slots 0..CAP-1 hold keys.
+//     K = u64        : slot 31 = valid bitmap (low 32 bits) | next slab (high 32)
+//     K = u32, map   : slot 31 = valid bitmap; next slab lives in value slot 31
+//     K = u32, set   : slot 30 = next slab, slot 31 = valid bitmap (CAP = 30)
+//   value block (maps): 32 slots of V; value slot i belongs to key slot i.
+// This keeps the reference's 31 keys per slab and its valid-bit semantics
+// (detail/slab_node.h:13-35) with a 32-bit slab index in place of the 8-byte
+// pointer, so a u64 key block is exactly 256 B (two 128-B lines) and a u32 key
+// block one line.  Slabs 0..2^n-1 are the bucket heads; pool slabs follow them
+// in the same array, so next == 0 means end of chain.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lslab_dev {
+
+enum : uint8_t { OP_GET = 0, OP_PUT = 1, OP_REMOVE = 2 };
+enum : uint32_t { ST_POOL_EXHAUSTED = 1u, ST_TIMEOUT = 2u, ST_CORRUPT = 4u };
+
+constexpr uint32_t LOCK_WRITER = 0x80000000u;
+constexpr uint64_t GT_EMPTY = ~0ull;
+constexpr uint64_t NOLOC = ~0ull;
+// 10 s at the 100 MHz constant clock: bound on every spin so that a lost
+// lock holder turns into a status code instead of a hung device.
+constexpr uint64_t SPIN_LIMIT_TICKS = 1000000000ull;
+
+template <typename K_, typename V_, bool MAP>
+struct Layout {
+    using K = K_;
+    using V = V_;
+    static constexpr bool IS_MAP = MAP;
+    static constexpr uint32_t KB = 32u * sizeof(K);
+    static constexpr uint32_t VB = MAP ? 32u * sizeof(V) : 0u;
+    static constexpr uint32_t SLAB_BYTES = KB + VB;
+    static constexpr int CAP = (sizeof(K) == 4 && !MAP) ? 30 : 31;
+    static constexpr uint32_t CAPMASK = (1u << CAP) - 1u;
+    // Byte offsets inside a slab.
+    static constexpr uint32_t VALID_OFF = (sizeof(K) == 8) ? 31u * 8u : 31u * 4u;
+    static constexpr uint32_t NEXT_OFF = (sizeof(K) == 8) ? 31u * 8u + 4u
+                                         : (MAP ? KB + 31u * sizeof(V) : 30u * 4u);
+};
+
+// Device view of one table (passed by value into kernels).
+struct TableDev {
+    char* slabs;           // (nbuckets + pool) * SLAB_BYTES, 256-B aligned
+    uint32_t* locks;       // one RW lock word per bucket
+    uint64_t* alloc_ctr;   // bump pointer into the pool (slabs handed out)
+    uint32_t* status;      // sticky ST_* bits
+    uint64_t* gt_key;      // batch combining table: key per entry (GT_EMPTY = free)
+    uint32_t* gt_head;     // last mutation (sub-batch op index + 1) per entry
+    uint32_t* op_gidx;     // per op: combining-table entry of its key
+    uint64_t* op_loc;      // per op: (slab << 5 | slot) of its key before the batch, or NOLOC
+    uint64_t nbuckets;
+    uint64_t pool;
+    uint32_t mask;         // nbuckets - 1
+    uint32_t gt_mask;      // gt capacity - 1
+};
+
+// Operation stream: per-op opcode (or a uniform one), keys and values with a
+// byte stride so that both SoA arrays and the reference's AoS pair<K,V> work.
+struct OpsDev {
+    const uint8_t* ops;    // null -> every op is `uop`
+    const char* keys;
+    const char* vals;      // may be null (gets, set ops)
+    uint32_t kstride, vstride;
+    uint8_t uop;
+};
+
+// Result sinks (any may be null): pair<bool,V> as the reference lays it out
+// (bool at 0, V at alignof(V)), a plain V array (reference map put) or a bool
+// array (reference set ops).
+struct OutDev {
+    char* pairs;
+    char* vals;
+    uint8_t* flags;
+};
+
+__device__ __forceinline__ uint64_t now_ticks() { return __builtin_amdgcn_s_memrealtime(); }
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z ^= z >> 33;
+    z *= 0xff51afd7ed558ccdull;
+    z ^= z >> 33;
+    z *= 0xc4ceb9fe1a85ec53ull;
+    z ^= z >> 33;
+    return z;
+}
+
+template <int G>
+struct Grp {
+    static_assert(G == 4 || G == 8 || G == 16 || G == 32, "lanes per op");
+    static constexpr int S = 32 / G;  // key slots per lane
+    __device__ static int lane() { return (int)(__lane_id() & (G - 1)); }
+    __device__ static int base() { return (int)(__lane_id() & ~(G - 1)); }
+    __device__ static uint64_t bits(uint64_t ballot) {
+        if constexpr (G == 64) return ballot;
+        else return (ballot >> base()) & ((1ull << G) - 1ull);
+    }
+};
+
+template <typename T>
+__device__ __forceinline__ T grp_bcast(T v, int src, int width) {
+    return __shfl(v, src, width);
+}
+__device__ __forceinline__ uint64_t grp_bcast(uint64_t v, int src, int width) {
+    uint32_t lo = __shfl((uint32_t)v, src, width);
+    uint32_t hi = __shfl((uint32_t)(v >> 32), src, width);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Loads `S` consecutive key slots starting at p (p is S*sizeof(K)-aligned).
+template <typename K, int S>
+__device__ __forceinline__ void load_slots(const K* p, K (&out)[S]) {
+    constexpr int BYTES = S * (int)sizeof(K);
+    if constexpr (BYTES >= 16) {
+        const uint4* q = reinterpret_cast<const uint4*>(p);
+#pragma unroll
+        for (int j = 0; j < BYTES / 16; ++j) {
+            uint4 w = q[j];
+            uint32_t tmp[4] = {w.x, w.y, w.z, w.w};
+            __builtin_memcpy(reinterpret_cast<char*>(out) + 16 * j, tmp, 16);
+        }
+    } else if constexpr (BYTES == 8) {
+        uint2 w = *reinterpret_cast<const uint2*>(p);
+        uint32_t tmp[2] = {w.x, w.y};
+        __builtin_memcpy(out, tmp, 8);
+    } else {
+        uint32_t w = *reinterpret_cast<const uint32_t*>(p);
+        __builtin_memcpy(out, &w, 4);
+    }
+}
+
+template <class L>
+__device__ __forceinline__ char* slab_ptr(const TableDev& t, uint64_t s) {
+    return t.slabs + s * (uint64_t)L::SLAB_BYTES;
+}
+template <class L>
+__device__ __forceinline__ uint32_t* valid_word(const TableDev& t, uint64_t s) {
+    return reinterpret_cast<uint32_t*>(slab_ptr<L>(t, s) + L::VALID_OFF);
+}
+template <class L>
+__device__ __forceinline__ uint32_t* next_word(const TableDev& t, uint64_t s) {
+    return reinterpret_cast<uint32_t*>(slab_ptr<L>(t, s) + L::NEXT_OFF);
+}
+template <class L>
+__device__ __forceinline__ typename L::K* key_slot(const TableDev& t, uint64_t s, int i) {
+    return reinterpret_cast<typename L::K*>(slab_ptr<L>(t, s)) + i;
+}
+template <class L>
+__device__ __forceinline__ typename L::V* val_slot(const TableDev& t, uint64_t s, int i) {
+    return reinterpret_cast<typename L::V*>(slab_ptr<L>(t, s) + L::KB) + i;
+}
+
+// Result of probing one slab with one group of G lanes.  All fields are
+// uniform across the group.
+struct Probe {
+    int slot;        // matching slot, or -1
+    uint32_t valid;  // valid bitmap of the slab
+    uint32_t next;   // next slab index (0 = last); only filled on a miss
+};
+
+// Group-cooperative probe of slab s for key.  Lane l of the group loads key
+// slots [l*S, l*S+S) in one coalesced access (a u64 key block is two 128-B
+// lines read by the whole group at once).  Plain loads: callers guarantee no
+// concurrent writer, or hold the bucket lock after an acquire.
+template <class L, int G>
+__device__ __forceinline__ Probe probe_slab(const TableDev& t, uint64_t s, typename L::K key) {
+    using K = typename L::K;
+    constexpr int S = Grp<G>::S;
+    const int gl = Grp<G>::lane();
+    const K* kb = reinterpret_cast<const K*>(slab_ptr<L>(t, s));
+    K mine[S];
+    load_slots<K, S>(kb + gl * S, mine);
+    Probe p;
+    if constexpr (sizeof(K) == 8) {
+        uint64_t meta = grp_bcast((uint64_t)mine[S - 1], G - 1, G);
+        p.valid = (uint32_t)meta;
+        p.next = (uint32_t)(meta >> 32);
+    } else if constexpr (!L::IS_MAP) {
+        p.valid = grp_bcast((uint32_t)mine[31 % S], 31 / S, G);
+        p.next = grp_bcast((uint32_t)mine[30 % S], 30 / S, G);
+    } else {
+        p.valid = grp_bcast((uint32_t)mine[S - 1], G - 1, G);
+        p.next = 0;
+    }
+    int hit = -1;
+#pragma unroll
+    for (int e = 0; e < S; ++e) {
+        int slot = gl * S + e;
+        if (slot < L::CAP && ((p.valid >> slot) & 1u) && mine[e] == key) hit = slot;
+    }
+    uint64_t b = Grp<G>::bits(__ballot(hit >= 0));
+    if (b) {
+        int src = __builtin_ctzll(b);
+        p.slot = grp_bcast(hit, src, G);
+    } else {
+        p.slot = -1;
+        if constexpr (sizeof(K) == 4 && L::IS_MAP) p.next = *next_word<L>(t, s);
+    }
+    return p;
+}
+
+// Walks the chain of bucket b: returns the slab/slot holding key, or slot -1
+// with `s` = last slab of the chain and `valid` its bitmap.
+template <class L, int G>
+__device__ __forceinline__ void walk(const TableDev& t, uint64_t b, typename L::K key,
+                                     uint64_t& s, int& slot, uint32_t& valid) {
+    s = b;
+    // A chain has at most pool + 1 slabs and every link points into the slab
+    // array; both are checked so that a damaged chain ends the walk with a
+    // status bit instead of a runaway loop or a wild access.
+    for (uint64_t hops = 0;; ++hops) {
+        Probe p = probe_slab<L, G>(t, s, key);
+        if (p.slot >= 0) {
+            slot = p.slot;
+            valid = p.valid;
+            return;
+        }
+        if (p.next == 0) {
+            slot = -1;
+            valid = p.valid;
+            return;
+        }
+        if (p.next < t.nbuckets || p.next >= t.nbuckets + t.pool || hops > t.pool) {
+            if (Grp<G>::lane() == 0)
+                __hip_atomic_fetch_or(t.status, ST_CORRUPT, __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
+            slot = -1;
+            valid = L::CAPMASK;  // report the last slab as full: no insert lands here
+            return;
+        }
+        s = p.next;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T ld_stride(const char* base, uint32_t stride, uint64_t i) {
+    return *reinterpret_cast<const T*>(base + i * (uint64_t)stride);
+}
+
+template <class L>
+__device__ __forceinline__ void write_result(const OutDev& o, uint64_t i, bool found,
+                                             typename L::V v) {
+    using V = typename L::V;
+    if (o.pairs) {
+        char* p = o.pairs + i * 2ull * sizeof(V);
+        *reinterpret_cast<uint8_t*>(p) = found ? 1 : 0;
+        *reinterpret_cast<V*>(p + sizeof(V)) = v;
+    }
+    if (o.vals) reinterpret_cast<V*>(o.vals)[i] = v;
+    if (o.flags) o.flags[i] = found ? 1 : 0;
+}
+
+__device__ __forceinline__ void acquire_agent() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+__device__ __forceinline__ void release_agent() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
+
+__device__ __forceinline__ uint32_t ld_agent(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t ld_agent(const uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Wave lock primitives on a bucket's 32-bit word (one lane calls them).
+// 0 = free, LOCK_WRITER = held exclusively, n = n readers.
+__device__ __forceinline__ bool try_lock_excl(uint32_t* w) {
+    uint32_t e = 0;
+    return __hip_atomic_compare_exchange_strong(w, &e, LOCK_WRITER, __ATOMIC_RELAXED,
+                                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ bool try_lock_shared(uint32_t* w) {
+    uint32_t old = __hip_atomic_fetch_add(w, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (old & LOCK_WRITER) {
+        __hip_atomic_fetch_add(w, 0xffffffffu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return false;
+    }
+    return true;
+}
+__device__ __forceinline__ void unlock_excl(uint32_t* w) {
+    __hip_atomic_fetch_and(w, ~LOCK_WRITER, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void unlock_shared(uint32_t* w) {
+    __hip_atomic_fetch_add(w, 0xffffffffu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+}  // namespace lslab_dev

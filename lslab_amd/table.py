@@ -1,0 +1,253 @@
+"""Python mirror of lslab::map<K,V> and lslab::set<K> over the C ABI.
+
+Method names, argument order and result meaning follow the reference's host
+batch API (include/lslab/map.h:189-220, include/lslab/set.h:92-123).  Device
+memory and streams come from torch (plumbing only): inputs and outputs are
+CUDA tensors, calls go on torch's current stream.  Keys and values are raw
+integer bit patterns: 4-byte widths use int32/uint32 tensors, 8-byte widths
+int64/uint64 tensors.
+"""
+import ctypes
+
+import torch
+
+from . import _lib
+
+_DT = {4: torch.int32, 8: torch.int64}
+
+
+def _stream(stream):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _dev(t, nbytes, what):
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise TypeError(f"{what} must be a CUDA tensor")
+    if t.element_size() != nbytes or t.is_floating_point():
+        raise TypeError(f"{what} must hold {nbytes}-byte integers, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{what} must be contiguous")
+    return t
+
+
+def _ops(t, n):
+    t = _dev(t, 1, "ops")
+    if t.numel() != n:
+        raise ValueError("ops and keys differ in length")
+    return t
+
+
+def tuning(lanes_per_op=16, engine=_lib.ENGINE_PLAIN, block_threads=256, max_blocks=0,
+           max_subbatch=1 << 20):
+    return _lib.Tuning(lanes_per_op, engine, block_threads, max_blocks, max_subbatch)
+
+
+class _Table:
+    is_map = False
+
+    def __init__(self):
+        self._h = None
+
+    def close(self):
+        if self._h:
+            _lib.check(_lib.load().lslab_destroy(self._h), "destroy")
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_tuning(self, tun):
+        _lib.check(_lib.load().lslab_set_tuning(self._h, ctypes.byref(tun)), "set_tuning")
+
+    def get_tuning(self):
+        t = _lib.Tuning()
+        _lib.check(_lib.load().lslab_get_tuning(self._h, ctypes.byref(t)), "get_tuning")
+        return t
+
+    def status(self, stream=None):
+        """Synchronises and returns the sticky device status (0 = ok), clearing it."""
+        return _lib.load().lslab_status(self._h, _stream(stream))
+
+    def stats(self, stream=None):
+        st = _lib.Stats()
+        _lib.check(_lib.load().lslab_stats_get(self._h, ctypes.byref(st), _stream(stream)), "stats")
+        return {f: getattr(st, f) for f, _ in _lib.Stats._fields_}
+
+    def buckets(self):
+        return 1 << self.n_log2
+
+
+class Map(_Table):
+    """lslab::map<K,V> (reference include/lslab/map.h:24-234)."""
+
+    is_map = True
+
+    def __init__(self, key_bytes=8, value_bytes=8, n_log2=10, pool_slabs=10000, tun=None):
+        super().__init__()
+        self.kb, self.vb, self.n_log2 = key_bytes, value_bytes, n_log2
+        h = ctypes.c_void_p()
+        _lib.check(_lib.load().lslab_map_create(ctypes.byref(h), key_bytes, value_bytes, n_log2,
+                                                pool_slabs, ctypes.byref(tun) if tun else None),
+                   "map_create")
+        self._h = h
+
+    def _pairs(self, n):
+        return torch.empty((n, 2), dtype=_DT[self.vb], device="cuda")
+
+    def split_pairs(self, raw):
+        """(found u8[n], value[n]) views of a pair<bool,V> result buffer."""
+        n = raw.shape[0]
+        found = raw.view(torch.uint8).view(n, 2 * self.vb)[:, 0]
+        return found, raw[:, 1]
+
+    def put(self, keys, values, out=None, stream=None):
+        """map::put(K*, V*, V* out, n, stream): out[i] = previous value (0 if absent)."""
+        n = keys.numel()
+        _dev(keys, self.kb, "keys")
+        _dev(values, self.vb, "values")
+        if out is None:
+            out = torch.empty(n, dtype=_DT[self.vb], device="cuda")
+        _lib.check(_lib.load().lslab_map_put(self._h, _ptr(keys), _ptr(values), _ptr(out), n,
+                                             _stream(stream)), "map_put")
+        return out
+
+    def put_pairs(self, pairs, out=None, stream=None):
+        """map::put(pair<K,V>*, V* out, n, stream); pairs is a byte tensor [n, 2*max(kb,vb)]."""
+        n = pairs.shape[0]
+        if out is None:
+            out = torch.empty(n, dtype=_DT[self.vb], device="cuda")
+        _lib.check(_lib.load().lslab_map_put_pairs(self._h, _ptr(pairs), _ptr(out), n,
+                                                   _stream(stream)), "map_put_pairs")
+        return out
+
+    def get(self, keys, stream=None, raw=False):
+        """map::get(K*, pair<bool,V>* out, n, stream) -> (found, value)."""
+        n = keys.numel()
+        _dev(keys, self.kb, "keys")
+        out = self._pairs(n)
+        _lib.check(_lib.load().lslab_map_get(self._h, _ptr(keys), _ptr(out), n, _stream(stream)),
+                   "map_get")
+        return out if raw else self.split_pairs(out)
+
+    def remove(self, keys, stream=None):
+        n = keys.numel()
+        _dev(keys, self.kb, "keys")
+        out = self._pairs(n)
+        _lib.check(_lib.load().lslab_map_remove(self._h, _ptr(keys), _ptr(out), n,
+                                                _stream(stream)), "map_remove")
+        return self.split_pairs(out)
+
+    def mixed(self, ops, keys, values, stream=None, raw=False):
+        n = keys.numel()
+        _ops(ops, n)
+        _dev(keys, self.kb, "keys")
+        _dev(values, self.vb, "values")
+        out = self._pairs(n)
+        _lib.check(_lib.load().lslab_map_mixed(self._h, _ptr(ops), _ptr(keys), _ptr(values),
+                                               _ptr(out), n, _stream(stream)), "map_mixed")
+        return out if raw else self.split_pairs(out)
+
+    def exec_batches(self, offsets, ops, keys, values, out=None, stream=None):
+        """Runs len(offsets)-1 batches in one persistent launch; returns (found, value)."""
+        n = keys.numel()
+        _ops(ops, n)
+        _dev(keys, self.kb, "keys")
+        _dev(values, self.vb, "values")
+        _dev(offsets, 8, "offsets")
+        if out is None:
+            out = self._pairs(n)
+        nb = offsets.numel() - 1
+        _lib.check(_lib.load().lslab_exec_batches(self._h, nb, _ptr(offsets), _ptr(ops),
+                                                  _ptr(keys), _ptr(values), _ptr(out),
+                                                  _stream(stream)), "exec_batches")
+        return self.split_pairs(out)
+
+    def dump(self, stream=None):
+        """All present (key, value) pairs, sorted by key (as unsigned)."""
+        st = self.stats(stream)
+        cap = max(st["present"], 1)
+        k = torch.empty(cap, dtype=_DT[self.kb], device="cuda")
+        v = torch.empty(cap, dtype=_DT[self.vb], device="cuda")
+        c = ctypes.c_uint64()
+        _lib.check(_lib.load().lslab_dump(self._h, _ptr(k), _ptr(v), cap, ctypes.byref(c),
+                                          _stream(stream)), "dump")
+        n = int(c.value)
+        return k[:n], v[:n]
+
+
+class Set(_Table):
+    """lslab::set<K> (reference include/lslab/set.h:21-123)."""
+
+    def __init__(self, key_bytes=8, n_log2=10, pool_slabs=10000, tun=None):
+        super().__init__()
+        self.kb, self.n_log2 = key_bytes, n_log2
+        h = ctypes.c_void_p()
+        _lib.check(_lib.load().lslab_set_create(ctypes.byref(h), key_bytes, n_log2, pool_slabs,
+                                                ctypes.byref(tun) if tun else None),
+                   "set_create")
+        self._h = h
+
+    def _call(self, fn, keys, stream):
+        n = keys.numel()
+        _dev(keys, self.kb, "keys")
+        out = torch.empty(n, dtype=torch.uint8, device="cuda")
+        _lib.check(fn(self._h, _ptr(keys), _ptr(out), n, _stream(stream)), fn.__name__)
+        return out
+
+    def insert(self, keys, stream=None):
+        return self._call(_lib.load().lslab_set_insert, keys, stream)
+
+    def contains(self, keys, stream=None):
+        return self._call(_lib.load().lslab_set_contains, keys, stream)
+
+    def remove(self, keys, stream=None):
+        return self._call(_lib.load().lslab_set_remove, keys, stream)
+
+    def mixed(self, ops, keys, stream=None):
+        n = keys.numel()
+        _ops(ops, n)
+        _dev(keys, self.kb, "keys")
+        out = torch.empty(n, dtype=torch.uint8, device="cuda")
+        _lib.check(_lib.load().lslab_set_mixed(self._h, _ptr(ops), _ptr(keys), _ptr(out), n,
+                                               _stream(stream)), "set_mixed")
+        return out
+
+    def exec_batches(self, offsets, ops, keys, out=None, stream=None):
+        n = keys.numel()
+        _ops(ops, n)
+        _dev(keys, self.kb, "keys")
+        _dev(offsets, 8, "offsets")
+        if out is None:
+            out = torch.empty(n, dtype=torch.uint8, device="cuda")
+        nb = offsets.numel() - 1
+        _lib.check(_lib.load().lslab_exec_batches(self._h, nb, _ptr(offsets), _ptr(ops),
+                                                  _ptr(keys), None, _ptr(out), _stream(stream)),
+                   "exec_batches")
+        return out
+
+    def dump(self, stream=None):
+        st = self.stats(stream)
+        cap = max(st["present"], 1)
+        k = torch.empty(cap, dtype=_DT[self.kb], device="cuda")
+        c = ctypes.c_uint64()
+        _lib.check(_lib.load().lslab_dump(self._h, _ptr(k), None, cap, ctypes.byref(c),
+                                          _stream(stream)), "dump")
+        return k[: int(c.value)]

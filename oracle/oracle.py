@@ -1,0 +1,137 @@
+"""ctypes front end of the lslab oracle (test infrastructure only).
+
+The oracle is a sequential host replay of the LSlab slab-list map and set
+(see lslab_oracle.c for the reference citations).  Only tests/, the smoke
+check and the benchmark's CPU-baseline leg import this module, and only to
+check or to time the host baseline; the product never does.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+OP_GET = 0
+OP_PUT = 1
+OP_REMOVE = 2
+
+
+def build():
+    """Compile the oracle's C sources with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE, "all"], check=True)
+
+
+def _lib():
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = os.path.join(_HERE, "liblslab_oracle.so")
+    if not os.path.exists(path):
+        build()
+    lib = ctypes.CDLL(path)
+    vp, u64, i64, u8p, u64p = (ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int64,
+                               ctypes.c_void_p, ctypes.c_void_p)
+    lib.orc_create.restype = vp
+    lib.orc_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint, u64,
+                               ctypes.c_int]
+    lib.orc_destroy.argtypes = [vp]
+    lib.orc_apply.restype = i64
+    lib.orc_apply.argtypes = [vp, u64, u8p, u64p, u64p, u8p, u64p]
+    lib.orc_lookup.restype = ctypes.c_int
+    lib.orc_lookup.argtypes = [vp, u64, ctypes.POINTER(ctypes.c_uint64)]
+    for f in ("orc_present", "orc_slabs_used", "orc_max_chain"):
+        getattr(lib, f).restype = u64
+        getattr(lib, f).argtypes = [vp]
+    lib.orc_dump.restype = u64
+    lib.orc_dump.argtypes = [vp, u64p, u64p, u64]
+    lib.orc_check_batch.restype = i64
+    lib.orc_check_batch.argtypes = [vp, u64, u8p, u64p, u64p, u8p, u64p, u8p, u64p,
+                                    ctypes.POINTER(ctypes.c_uint64)]
+    _LIB = lib
+    return lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _u64(a, n):
+    a = np.ascontiguousarray(a).astype(np.uint64, copy=False)
+    assert a.shape == (n,)
+    return a
+
+
+class Replay:
+    """Sequential replay table: map (is_map=True) or set."""
+
+    def __init__(self, is_map, key_bytes, val_bytes, n_log2, pool_slabs=10000, slab_keys=31):
+        self.is_map = bool(is_map)
+        self.h = _lib().orc_create(int(is_map), key_bytes, val_bytes if is_map else 0,
+                                   n_log2, pool_slabs, slab_keys)
+        if not self.h:
+            raise ValueError("orc_create rejected the configuration")
+
+    def close(self):
+        if self.h:
+            _lib().orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def apply(self, ops, keys, vals=None):
+        """Apply ops in index order; returns (found u8[n], val u64[n], failed)."""
+        n = len(keys)
+        ops = np.ascontiguousarray(ops, dtype=np.uint8)
+        keys = _u64(keys, n)
+        vals = None if vals is None else _u64(vals, n)
+        found = np.zeros(n, np.uint8)
+        out = np.zeros(n, np.uint64)
+        failed = _lib().orc_apply(self.h, n, _p(ops), _p(keys), _p(vals), _p(found), _p(out))
+        return found, out, int(failed)
+
+    def lookup(self, key):
+        v = ctypes.c_uint64(0)
+        f = _lib().orc_lookup(self.h, int(key), ctypes.byref(v))
+        return bool(f), int(v.value)
+
+    @property
+    def present(self):
+        return int(_lib().orc_present(self.h))
+
+    @property
+    def slabs_used(self):
+        return int(_lib().orc_slabs_used(self.h))
+
+    @property
+    def max_chain(self):
+        return int(_lib().orc_max_chain(self.h))
+
+    def dump(self):
+        n = self.present
+        k = np.zeros(max(n, 1), np.uint64)
+        v = np.zeros(max(n, 1), np.uint64)
+        c = _lib().orc_dump(self.h, _p(k), _p(v), n)
+        assert c == n
+        return k[:n], v[:n]
+
+    def check_batch(self, ops, keys, vals, found, got, post_found=None, post_val=None):
+        """Serial-order check of observed results; adopts the end states.
+
+        Returns (number of keys with no valid serial order, first bad op index
+        or None)."""
+        n = len(keys)
+        ops = np.ascontiguousarray(ops, dtype=np.uint8)
+        keys = _u64(keys, n)
+        vals = _u64(vals if vals is not None else np.zeros(n, np.uint64), n)
+        found = np.ascontiguousarray(found, dtype=np.uint8)
+        got = _u64(got, n)
+        pf = None if post_found is None else np.ascontiguousarray(post_found, dtype=np.uint8)
+        pv = None if post_val is None else _u64(post_val, n)
+        fb = ctypes.c_uint64(0)
+        bad = _lib().orc_check_batch(self.h, n, _p(ops), _p(keys), _p(vals), _p(found),
+                                     _p(got), _p(pf), _p(pv), ctypes.byref(fb))
+        return int(bad), (None if fb.value == 2**64 - 1 else int(fb.value))

@@ -1,0 +1,66 @@
+"""CPU checks of the drop-in boundary: liblslab.so loads and exports every
+symbol the headers in include/ declare (no compute calls without a GPU)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from lslab_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared(header):
+    src = open(os.path.join(ROOT, "include", header)).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(lslab_\w+)\s*\(", src, re.M)))
+
+
+@pytest.fixture(scope="module")
+def lib(lslab):
+    return _lib.load()
+
+
+def test_every_declared_symbol_is_exported(lib):
+    names = _declared("lslab_c.h") + _declared("lslab_workload.h")
+    assert len(names) >= 23
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(_declared("lslab_c.h")) == set(_lib.EXPORTS)
+
+
+def test_error_strings(lib):
+    for code in range(0, 7):
+        assert _lib.error_string(code) not in ("", "unknown status")
+    assert _lib.error_string(99) == "unknown status"
+
+
+def test_invalid_arguments_rejected_without_device_work(lib):
+    h = ctypes.c_void_p()
+    # widths other than 4/8 and n_log2 > 30 are refused before any allocation
+    assert lib.lslab_map_create(ctypes.byref(h), 2, 8, 10, 10, None) == _lib.ERR_INVALID
+    assert lib.lslab_map_create(ctypes.byref(h), 8, 8, 31, 10, None) == _lib.ERR_INVALID
+    assert lib.lslab_set_create(ctypes.byref(h), 16, 10, 10, None) == _lib.ERR_INVALID
+    bad = _lib.Tuning(12, 0, 256, 0, 1 << 20)
+    assert lib.lslab_map_create(ctypes.byref(h), 8, 8, 10, 10, ctypes.byref(bad)) == _lib.ERR_INVALID
+    assert lib.lslab_destroy(None) == _lib.ERR_INVALID
+    assert lib.lslab_map_get(None, None, None, 0, None) == _lib.ERR_INVALID
+
+
+def test_cpp_headers_compile():
+    """The header-only C++ API compiles on its own (host-only translation unit)."""
+    import subprocess
+    import tempfile
+    src = ('#include <lslab/map.h>\n#include <lslab/set.h>\n'
+           'int main(){ static_assert(sizeof(lslab::found_value<unsigned long long>)==16,"");'
+           ' static_assert(sizeof(lslab::found_value<unsigned>)==8,"");'
+           ' static_assert(sizeof(lslab::key_value<unsigned,unsigned long long>)==16,"");'
+           ' return 0; }\n')
+    with tempfile.TemporaryDirectory() as d:
+        f = os.path.join(d, "t.cpp")
+        open(f, "w").write(src)
+        r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-D__HIP_PLATFORM_AMD__",
+                            "-I", os.path.join(ROOT, "include"), "-I", "/opt/rocm/include", f],
+                           capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr

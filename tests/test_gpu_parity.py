@@ -1,0 +1,302 @@
+"""GPU parity: the HIP map/set against the oracle (sequential replay).
+
+Phase-separated batches (no two ops of a batch on one key) must match the
+oracle bit for bit, op by op, and so must the final table contents and the
+slab accounting.  Batches with same-key conflicts are checked per key against
+SOME serial order by the oracle's checker.  Every case names itself on stdout
+before it touches the GPU.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle.oracle import OP_GET, OP_PUT, OP_REMOVE, Replay
+
+pytestmark = pytest.mark.gpu
+
+TYPES = [(True, 4, 4), (True, 4, 8), (True, 8, 4), (True, 8, 8), (False, 4, 0), (False, 8, 0)]
+ENGINES = [0, 1]
+LANES = [4, 8, 16, 32]
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+def _np_dt(b):
+    return np.int32 if b == 4 else np.int64
+
+
+def to_dev(a, b):
+    """uint64 numpy bit patterns -> CUDA int tensor of width b."""
+    a = np.asarray(a, dtype=np.uint64)
+    if b == 4:
+        a = (a & np.uint64(0xFFFFFFFF)).astype(np.uint32).view(np.int32)
+    else:
+        a = a.view(np.int64)
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def to_host(t, b):
+    a = t.cpu().numpy()
+    return a.view(np.uint32).astype(np.uint64) if b == 4 else a.view(np.uint64)
+
+
+def make(lslab, is_map, kb, vb, n_log2, pool, engine=0, lanes=16, subbatch=1 << 16):
+    tun = lslab.tuning(lanes_per_op=lanes, engine=engine, max_subbatch=subbatch)
+    if is_map:
+        return lslab.Map(kb, vb, n_log2, pool, tun)
+    return lslab.Set(kb, n_log2, pool, tun)
+
+
+def run_batch(tbl, is_map, kb, vb, ops, keys, vals):
+    """Runs a mixed batch; returns (found u8, val u64) on the host."""
+    n = len(keys)
+    d_ops = torch.from_numpy(np.ascontiguousarray(ops, dtype=np.uint8)).cuda()
+    d_keys = to_dev(keys, kb)
+    if is_map:
+        d_vals = to_dev(vals, vb)
+        f, v = tbl.mixed(d_ops, d_keys, d_vals)
+        return f.cpu().numpy().astype(np.uint8), to_host(v.contiguous(), vb)
+    f = tbl.mixed(d_ops, d_keys)
+    return f.cpu().numpy().astype(np.uint8), np.zeros(n, np.uint64)
+
+
+def dump_sorted(tbl, is_map, kb, vb):
+    if is_map:
+        k, v = tbl.dump()
+        k, v = to_host(k, kb), to_host(v, vb)
+    else:
+        k = to_host(tbl.dump(), kb)
+        v = np.zeros(len(k), np.uint64)
+    o = np.argsort(k, kind="stable")
+    return k[o], v[o]
+
+
+def oracle_dump_sorted(r):
+    k, v = r.dump()
+    o = np.argsort(k, kind="stable")
+    return k[o], v[o]
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+@pytest.mark.parametrize("lanes", [8, 16])
+def test_kat_get_put(lslab, engine, lanes):
+    # test/slabunified_test.cu:42-89: map<int,int>(10), keys 1..1024, 100 reps.
+    say("kat_get_put", engine, lanes)
+    m = make(lslab, True, 4, 4, 10, 10000, engine, lanes)
+    keys = torch.arange(1, 1025, dtype=torch.int32, device="cuda")
+    for rep in range(100):
+        vals = torch.full((1024,), rep, dtype=torch.int32, device="cuda")
+        old = m.put(keys, vals)
+        if rep:
+            assert bool((old == rep - 1).all()), rep
+        f, v = m.get(keys)
+        assert bool((f == 1).all()) and bool((v == rep).all()), rep
+    assert m.status() == 0
+
+
+def test_kat_get_put_pairs(lslab):
+    # map::put(pair<K,V>*, ...) (map.h:189-192) with the pair layout.
+    say("kat_get_put_pairs")
+    m = make(lslab, True, 4, 8, 10, 10000)
+    n = 1024
+    for rep in range(3):
+        raw = torch.zeros((n, 2), dtype=torch.int64, device="cuda")
+        raw.view(torch.int32)[:, 0] = torch.arange(1, n + 1, dtype=torch.int32, device="cuda")
+        raw[:, 1] = rep + (1 << 40)
+        old = m.put_pairs(raw.view(torch.uint8).view(n, 16))
+        if rep:
+            assert bool((old == rep - 1 + (1 << 40)).all())
+        f, v = m.get(torch.arange(1, n + 1, dtype=torch.int32, device="cuda"))
+        assert bool((f == 1).all()) and bool((v == rep + (1 << 40)).all())
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+def test_kat_contains_insert_remove(lslab, engine):
+    # test/set_test.cu:65-106: set<int>(10), keys 1..1024, 100 reps.
+    say("kat_contains_insert_remove", engine)
+    s = make(lslab, False, 4, 0, 10, 10000, engine)
+    keys = torch.arange(1, 1025, dtype=torch.int32, device="cuda")
+    for rep in range(100):
+        assert bool((s.insert(keys) == 1).all())
+        assert bool((s.contains(keys) == 1).all())
+        assert bool((s.remove(keys) == 1).all())
+        assert bool((s.contains(keys) == 0).all())
+    assert s.status() == 0
+
+
+@pytest.mark.parametrize("is_map,kb,vb", TYPES)
+@pytest.mark.parametrize("engine", ENGINES)
+@pytest.mark.parametrize("lanes", LANES)
+def test_phase_separated_bit_exact(lslab, oracle_mod, is_map, kb, vb, engine, lanes):
+    """Distinct keys per batch, 16 buckets (chains of many slabs), puts,
+    gets of present and absent keys, removes, re-puts into freed slots."""
+    say("phase_separated", is_map, kb, vb, engine, lanes)
+    rng = np.random.default_rng(100 + kb + vb + lanes + 7 * engine)
+    n_log2 = 4
+    cap = 30 if (kb == 4 and not is_map) else 31
+    tbl = make(lslab, is_map, kb, vb, n_log2, 4096, engine, lanes)
+    ref = Replay(is_map, kb, vb, n_log2, pool_slabs=4096, slab_keys=cap)
+    universe = rng.permutation(np.arange(1, 6001, dtype=np.uint64) * np.uint64(2654435761))
+    if kb == 4:
+        universe = universe & np.uint64(0xFFFFFFFF)
+        universe = np.unique(universe)
+        rng.shuffle(universe)
+    batches = []
+    a = universe[:3000]
+    batches.append((np.full(3000, OP_PUT, np.uint8), a))
+    batches.append((np.zeros(4000, np.uint8), np.concatenate([a[:2000], universe[3000:5000]])))
+    batches.append((np.full(1500, OP_REMOVE, np.uint8), rng.permutation(a)[:1500]))
+    batches.append((np.zeros(3000, np.uint8), a))
+    b = np.concatenate([a[:1000], universe[5000:5500]])  # updates + new keys into freed slots
+    batches.append((np.full(len(b), OP_PUT, np.uint8), b))
+    mixed_keys = rng.permutation(universe[:5600])[:2500]  # one op per key
+    batches.append((rng.choice([0, 1, 2], size=2500).astype(np.uint8), mixed_keys))
+    for bi, (ops, keys) in enumerate(batches):
+        vals = rng.integers(1, 2**63, size=len(keys), dtype=np.uint64)
+        if vb == 4:
+            vals &= np.uint64(0xFFFFFFFF)
+        f_ref, v_ref, fail = ref.apply(ops, keys, vals)
+        assert fail == 0
+        f, v = run_batch(tbl, is_map, kb, vb, ops, keys, vals)
+        bad = np.flatnonzero((f != f_ref) | (v != v_ref))
+        assert len(bad) == 0, (bi, bad[:5], f[bad[:5]], f_ref[bad[:5]], v[bad[:5]], v_ref[bad[:5]])
+    assert tbl.status() == 0
+    k, vv = dump_sorted(tbl, is_map, kb, vb)
+    rk, rv = oracle_dump_sorted(ref)
+    np.testing.assert_array_equal(k, rk)
+    np.testing.assert_array_equal(vv, rv)
+    st = tbl.stats()
+    assert st["present"] == ref.present
+    if engine == 0:
+        # The plain engine applies a phase-separated batch's inserts with the
+        # reference's placement rule, so pool use matches exactly.
+        assert st["slabs_used"] == ref.slabs_used
+    assert st["max_chain"] >= 2
+
+
+@pytest.mark.parametrize("is_map,kb,vb", [(True, 8, 8), (True, 4, 4), (False, 8, 0)])
+@pytest.mark.parametrize("engine", ENGINES)
+def test_conflicting_batches_serial_order(lslab, oracle_mod, is_map, kb, vb, engine):
+    """Zipf-like hot keys with get/put/remove conflicts inside each batch:
+    every key's results must match some serial order of its ops."""
+    say("conflicting", is_map, kb, vb, engine)
+    rng = np.random.default_rng(5 + engine)
+    n_log2 = 5
+    tbl = make(lslab, is_map, kb, vb, n_log2, 1 << 14, engine, 16)
+    ref = Replay(is_map, kb, vb, n_log2, pool_slabs=1 << 14)
+    space = 3000
+    p = 1.0 / np.arange(1, space + 1) ** 1.1
+    p /= p.sum()
+    perm = rng.permutation(space).astype(np.uint64) * np.uint64(0x9E3779B1) + np.uint64(1)
+    if kb == 4:
+        perm &= np.uint64(0xFFFFFFFF)
+    for bi in range(6):
+        n = 20000
+        keys = perm[rng.choice(space, size=n, p=p)]
+        ops = rng.choice([0, 1, 2], size=n, p=[0.6, 0.3, 0.1]).astype(np.uint8)
+        vals = rng.integers(1, 2**63, size=n, dtype=np.uint64)
+        if vb == 4:
+            vals &= np.uint64(0xFFFFFFFF)
+        f, v = run_batch(tbl, is_map, kb, vb, ops, keys, vals)
+        pf, pv = run_batch(tbl, is_map, kb, vb, np.zeros(n, np.uint8), keys, vals)
+        bad, first = ref.check_batch(ops, keys, vals, f, v, pf, pv)
+        assert bad == 0, (bi, first)
+    assert tbl.status() == 0
+    k, vv = dump_sorted(tbl, is_map, kb, vb)
+    rk, rv = oracle_dump_sorted(ref)
+    np.testing.assert_array_equal(k, rk)
+    np.testing.assert_array_equal(vv, rv)
+
+
+def test_pool_exhaustion_status(lslab, oracle_mod):
+    # 1 bucket, pool of 2 slabs: 93 keys fit, 5 inserts fail with a status.
+    say("pool_exhaustion")
+    s = make(lslab, False, 8, 0, 0, 2)
+    keys = torch.arange(98, dtype=torch.int64, device="cuda")
+    res = s.insert(keys)
+    assert s.status() == 4
+    assert int(res.sum()) == 93
+    st = s.stats()
+    assert st["present"] == 93 and st["slabs_used"] == 2
+    assert s.status() == 0  # cleared
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+def test_plain_small_config(lslab, oracle_mod, engine):
+    """2^20 distinct uint32 keys, 2^16 buckets, one put batch, then 2^20 gets
+    (half present, half absent): bit-exact against the replay."""
+    say("plain_small", engine)
+    from lslab_amd import workload
+    n = 1 << 20
+    keys = workload.keys(n, 4, first=0, seed=0)
+    vals = workload.values(n, 4, first=0, seed=1)
+    absent = workload.keys(n // 2, 4, first=n, seed=0)
+    present = keys[torch.randperm(n, device="cuda")[: n // 2]]
+    q = torch.cat([present, absent])[torch.randperm(n, device="cuda")]
+    m = make(lslab, True, 4, 4, 16, 10000, engine)
+    old = m.put(keys, vals)
+    f, v = m.get(q)
+    assert m.status() == 0
+    ref = Replay(True, 4, 4, 16)
+    hk, hv, hq = to_host(keys, 4), to_host(vals, 4), to_host(q, 4)
+    rf, rold, fail = ref.apply(np.full(n, OP_PUT, np.uint8), hk, hv)
+    assert fail == 0
+    np.testing.assert_array_equal(to_host(old, 4), rold)
+    gf, gv, _ = ref.apply(np.zeros(n, np.uint8), hq)
+    np.testing.assert_array_equal(f.cpu().numpy(), gf)
+    np.testing.assert_array_equal(to_host(v.contiguous(), 4), gv)
+    assert int(gf.sum()) == n // 2
+    st = m.stats()
+    assert st["present"] == n
+    if engine == 0:
+        assert st["slabs_used"] == ref.slabs_used
+
+
+@pytest.mark.parametrize("is_map", [True, False])
+def test_exec_batches_uneven(lslab, oracle_mod, is_map):
+    """Uneven batches in one persistent launch; each batch checked in order."""
+    say("exec_batches", is_map)
+    rng = np.random.default_rng(77)
+    kb, vb = 8, 8
+    tbl = make(lslab, is_map, kb, vb, 6, 1 << 14, 1, 16, subbatch=1 << 12)
+    ref = Replay(is_map, kb, vb, 6, pool_slabs=1 << 14)
+    sizes = np.exp(rng.uniform(np.log(1), np.log(9000), size=40)).astype(np.int64)
+    sizes[3] = 0  # an empty batch
+    n = int(sizes.sum())
+    space = 5000
+    p = 1.0 / np.arange(1, space + 1) ** 0.99
+    p /= p.sum()
+    if is_map:
+        ranks = rng.choice(space, size=n, p=p)
+    else:
+        # A set's state after a batch is not implied by its results, so each
+        # set batch touches distinct keys (results are then deterministic).
+        ranks = np.concatenate([rng.permutation(20000)[:s] for s in sizes])
+    keys = ranks.astype(np.uint64) * np.uint64(0x9E3779B97F4A7C15)
+    ops = rng.choice([0, 1, 2], size=n, p=[0.7, 0.25, 0.05]).astype(np.uint8)
+    vals = rng.integers(1, 2**63, size=n, dtype=np.uint64)
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    d_offs = torch.from_numpy(offs).cuda()
+    d_ops = torch.from_numpy(ops).cuda()
+    if is_map:
+        f, v = tbl.exec_batches(d_offs, d_ops, to_dev(keys, kb), to_dev(vals, vb))
+        f, v = f.cpu().numpy(), to_host(v.contiguous(), vb)
+    else:
+        f = tbl.exec_batches(d_offs, d_ops, to_dev(keys, kb)).cpu().numpy()
+        v = np.zeros(n, np.uint64)
+    assert tbl.status() == 0
+    # Replay batch by batch; the end state of each key after a batch is not
+    # observable mid-stream for sets, so sets use the checker without it.
+    for b in range(len(sizes)):
+        lo, hi = offs[b], offs[b + 1]
+        if hi == lo:
+            continue
+        bad, first = ref.check_batch(ops[lo:hi], keys[lo:hi], vals[lo:hi], f[lo:hi], v[lo:hi])
+        assert bad == 0, (b, first)
+    if is_map:
+        k, vv = dump_sorted(tbl, is_map, kb, vb)
+        rk, rv = oracle_dump_sorted(ref)
+        np.testing.assert_array_equal(k, rk)
+        np.testing.assert_array_equal(vv, rv)
