@@ -134,6 +134,40 @@ int lslab_exec_batches(lslab_table* t, size_t nbatches, const uint64_t* offsets,
                        const uint8_t* ops, const void* keys, const void* values, void* out,
                        hipStream_t stream);
 
+/* Served form (new): a device-resident request loop.  Request r owns ops
+ * [req_offsets[r], req_offsets[r+1]) of ops/keys/values/out and arrives
+ * arrival_ticks[r] ticks of the device's 100 MHz constant clock after the
+ * loop starts (nondecreasing).  One persistent launch admits arrived requests
+ * in order, tick by tick (up to max_tick_ops ops and max_tick_reqs requests
+ * per tick, and only while the slab pool has headroom for req_reserve[r]
+ * slabs -- one per put bounds what a request can allocate; NULL reserves one
+ * per op), runs each tick as one two-phase batch, and writes
+ * complete_ticks[r] (same clock) and tick_of[r] (LSLAB_SERVE_REJECTED when the
+ * pool can never hold the request).  All arrays are DEVICE memory. */
+#define LSLAB_SERVE_REJECTED 0xffffffffu
+typedef struct {
+    uint64_t nreq;
+    const uint64_t* req_offsets;   /* nreq + 1 */
+    const uint64_t* arrival_ticks; /* nreq */
+    const uint32_t* req_reserve;   /* nreq, or NULL */
+    uint64_t* complete_ticks;      /* nreq, written */
+    uint32_t* tick_of;             /* nreq, written */
+    uint64_t max_tick_ops;         /* 0 = the table's max_subbatch */
+    uint32_t max_tick_reqs;        /* 0 = no limit */
+} lslab_serve_params;
+
+typedef struct {
+    uint64_t ticks;         /* batches run */
+    uint64_t rejected;      /* requests refused for lack of pool headroom */
+    uint64_t served_ops;    /* ops admitted */
+    uint64_t elapsed_ticks; /* loop duration on the 100 MHz clock */
+} lslab_serve_stats;
+
+int lslab_serve(lslab_table* t, const lslab_serve_params* p, const uint8_t* ops, const void* keys,
+                const void* values, void* out, hipStream_t stream);
+/* Synchronises and reads the last serve loop's counters. */
+int lslab_serve_result(lslab_table* t, lslab_serve_stats* out, hipStream_t stream);
+
 /* Synchronises the stream, returns the table's sticky device status
  * (LSLAB_OK, LSLAB_ERR_POOL_EXHAUSTED, LSLAB_ERR_TIMEOUT) and clears it. */
 int lslab_status(lslab_table* t, hipStream_t stream);

@@ -29,8 +29,10 @@ EXPORTS = (
     "lslab_get_tuning", "lslab_map_put_pairs", "lslab_map_put", "lslab_map_get",
     "lslab_map_remove", "lslab_map_mixed", "lslab_set_insert", "lslab_set_contains",
     "lslab_set_remove", "lslab_set_mixed", "lslab_exec_batches", "lslab_status",
-    "lslab_stats_get", "lslab_dump", "lslab_error_string",
+    "lslab_stats_get", "lslab_dump", "lslab_error_string", "lslab_serve", "lslab_serve_result",
 )
+
+SERVE_REJECTED = 0xFFFFFFFF
 
 
 class Tuning(ctypes.Structure):
@@ -52,6 +54,28 @@ class Stats(ctypes.Structure):
         ("max_chain", ctypes.c_uint64),
         ("slab_bytes", ctypes.c_uint64),
         ("device_bytes", ctypes.c_uint64),
+    ]
+
+
+class ServeParams(ctypes.Structure):
+    _fields_ = [
+        ("nreq", ctypes.c_uint64),
+        ("req_offsets", ctypes.c_void_p),
+        ("arrival_ticks", ctypes.c_void_p),
+        ("req_reserve", ctypes.c_void_p),
+        ("complete_ticks", ctypes.c_void_p),
+        ("tick_of", ctypes.c_void_p),
+        ("max_tick_ops", ctypes.c_uint64),
+        ("max_tick_reqs", ctypes.c_uint32),
+    ]
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [
+        ("ticks", ctypes.c_uint64),
+        ("rejected", ctypes.c_uint64),
+        ("served_ops", ctypes.c_uint64),
+        ("elapsed_ticks", ctypes.c_uint64),
     ]
 
 
@@ -94,6 +118,8 @@ def load():
         "lslab_status": [vp, vp],
         "lslab_stats_get": [vp, ctypes.POINTER(Stats), vp],
         "lslab_dump": [vp, vp, vp, u64, ctypes.POINTER(ctypes.c_uint64), vp],
+        "lslab_serve": [vp, ctypes.POINTER(ServeParams), vp, vp, vp, vp, vp],
+        "lslab_serve_result": [vp, ctypes.POINTER(ServeStats), vp],
     }
     for name, args in sig.items():
         f = getattr(lib, name)

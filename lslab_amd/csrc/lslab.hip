@@ -128,6 +128,133 @@ __global__ void k_reset_bar(BarState* bar) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Served form: a device-resident request loop.
+//
+// Requests r = 0..nreq-1 own the contiguous op ranges [off[r], off[r+1]) and
+// arrive at arrival[r] (ticks of the 100 MHz constant clock after the loop
+// starts, nondecreasing).  Each tick, one scheduler lane admits, in arrival
+// order, every request that has arrived, up to max_tick_ops ops and
+// max_tick_reqs requests, provided the slab pool keeps headroom for it: a
+// request reserves one slab per put, the most its puts can allocate (an
+// insert allocates at most one slab).  A request whose reservation can never
+// fit (the pool only shrinks) is rejected.  All blocks then run the admitted
+// ops as one two-phase batch; the scheduler stamps each request's completion
+// time.  Requests arriving while a tick runs join the next tick.
+// ---------------------------------------------------------------------------
+
+struct ServeDev {
+    uint64_t nreq;
+    const uint64_t* off;
+    const uint64_t* arrival;
+    const uint32_t* reserve;
+    uint64_t* complete;
+    uint32_t* tick_of;
+    uint64_t max_tick_ops;
+    uint32_t max_tick_reqs;
+};
+
+struct ServeState {
+    uint64_t t0;
+    uint64_t next_req;
+    uint64_t r_lo, r_hi;
+    uint64_t ticks;
+    uint64_t rejected;
+    uint64_t served_ops;
+    uint64_t end;
+    uint32_t done;
+    uint32_t pad;
+};
+
+constexpr uint32_t SERVE_REJECTED = 0xffffffffu;
+
+__device__ void serve_schedule(const TableDev& t, const ServeDev& sv, ServeState* ss) {
+    uint64_t r = ss->next_req;
+    const uint64_t used = ld_agent(t.alloc_ctr);
+    uint64_t reserved = 0, ops = 0, n = 0, lo = r;
+    while (r < sv.nreq) {
+        const uint64_t el = now_ticks() - ss->t0;
+        if (sv.arrival[r] > el) {
+            if (n > 0) break;
+            // Idle: wait for the next arrival (the gap is bounded by the caller).
+            const uint64_t start = now_ticks();
+            while (now_ticks() - ss->t0 < sv.arrival[r] && now_ticks() - start < SPIN_LIMIT_TICKS / 2)
+                __builtin_amdgcn_s_sleep(2);
+            continue;
+        }
+        const uint64_t sz = sv.off[r + 1] - sv.off[r];
+        if (n > 0 && (ops + sz > sv.max_tick_ops || n >= sv.max_tick_reqs)) break;
+        const uint64_t need = sv.reserve ? sv.reserve[r] : sz;
+        if (used + reserved + need > t.pool) {
+            if (n > 0) break;
+            sv.tick_of[r] = SERVE_REJECTED;
+            sv.complete[r] = now_ticks() - ss->t0;
+            ss->rejected++;
+            r++;
+            lo = r;
+            continue;
+        }
+        reserved += need;
+        ops += sz;
+        n++;
+        r++;
+    }
+    ss->r_lo = lo;
+    ss->r_hi = r;
+    ss->next_req = r;
+    ss->served_ops += ops;
+    if (n == 0) ss->done = 1;
+}
+
+template <class L, int G>
+__global__ __launch_bounds__(256) void k_serve(TableDev t, OpsDev o, OutDev out, ServeDev sv,
+                                               ServeState* ss, BarState* bar, uint64_t submax) {
+    const uint64_t ng = (uint64_t)gridDim.x * blockDim.x / G;
+    const uint64_t gid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+    __shared__ uint64_t stamp_s;
+    if (blockIdx.x == 0 && threadIdx.x == 0) ss->t0 = now_ticks();
+    for (uint64_t tick = 0;; ++tick) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) serve_schedule(t, sv, ss);
+        if (!grid_sync(bar, gridDim.x)) goto fail;
+        {
+            if (ld_agent(&ss->done)) break;
+            const uint64_t r_lo = ss->r_lo, r_hi = ss->r_hi;
+            const uint64_t lo = sv.off[r_lo], hi = sv.off[r_hi];
+            for (uint64_t slo = lo; slo < hi; slo += submax) {
+                const uint64_t m = min(submax, hi - slo);
+                for (uint64_t li = gid; li < m; li += ng) phase_r_op<L, G>(t, o, out, slo, (uint32_t)li);
+                if (!grid_sync(bar, gridDim.x)) goto fail;
+                for (uint64_t li = gid; li < m; li += ng) phase_w_op<L, G>(t, o, slo, (uint32_t)li);
+                if (!grid_sync(bar, gridDim.x)) goto fail;
+            }
+            if (blockIdx.x == 0) {
+                if (threadIdx.x == 0) stamp_s = now_ticks() - ss->t0;
+                __syncthreads();
+                for (uint64_t r = r_lo + threadIdx.x; r < r_hi; r += blockDim.x) {
+                    sv.complete[r] = stamp_s;
+                    sv.tick_of[r] = (uint32_t)tick;
+                }
+                __syncthreads();
+                if (threadIdx.x == 0) ss->ticks = tick + 1;
+            }
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) ss->end = now_ticks() - ss->t0;
+    return;
+fail:
+    if (threadIdx.x == 0)
+        __hip_atomic_fetch_or(t.status, ST_TIMEOUT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void k_reset_serve(ServeState* ss, BarState* bar) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        ServeState z{};
+        *ss = z;
+        bar->count = 0;
+        bar->abort = 0;
+    }
+}
+
 // Introspection: one thread per bucket walks its chain.
 struct StatsDev {
     unsigned long long present;
@@ -203,6 +330,7 @@ struct DevState {
     BarState bar;
     StatsDev stats;
     unsigned long long dump_counter;
+    ServeState serve;
 };
 
 enum TableType { T_MAP_32_32, T_MAP_32_64, T_MAP_64_32, T_MAP_64_64, T_SET_32, T_SET_64 };
@@ -438,13 +566,19 @@ extern "C" int lslab_get_tuning(const lslab_table* t, lslab_tuning* out) {
 }
 
 // Orders this call after the table's previous call, whatever its stream.
+// While the stream is being captured into a graph the graph's own edges order
+// the calls, so the table's event is neither waited on nor recorded.
+static bool capturing(hipStream_t s) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+}
 static int begin_call(lslab_table* t, hipStream_t s) {
-    HIPCHK(hipStreamWaitEvent(s, t->order_ev, 0));
+    if (!capturing(s)) HIPCHK(hipStreamWaitEvent(s, t->order_ev, 0));
     return LSLAB_OK;
 }
 static int end_call(lslab_table* t, hipStream_t s) {
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(t->order_ev, s));
+    if (!capturing(s)) HIPCHK(hipEventRecord(t->order_ev, s));
     return LSLAB_OK;
 }
 
@@ -603,6 +737,63 @@ extern "C" int lslab_exec_batches(lslab_table* t, size_t nbatches, const uint64_
     });
     if (rc) return rc;
     return end_call(t, stream);
+}
+
+extern "C" int lslab_serve(lslab_table* t, const lslab_serve_params* p, const uint8_t* ops,
+                           const void* keys, const void* values, void* out, hipStream_t stream) {
+    if (!t || !p || (p->nreq && (!p->req_offsets || !p->arrival_ticks || !p->complete_ticks ||
+                                 !p->tick_of || !ops || !keys || !out)))
+        return LSLAB_ERR_INVALID;
+    if (t->is_map && p->nreq && !values) return LSLAB_ERR_INVALID;
+    int rc = begin_call(t, stream);
+    if (rc) return rc;
+    ServeDev sv;
+    sv.nreq = p->nreq;
+    sv.off = p->req_offsets;
+    sv.arrival = p->arrival_ticks;
+    sv.reserve = p->req_reserve;
+    sv.complete = p->complete_ticks;
+    sv.tick_of = p->tick_of;
+    sv.max_tick_ops = p->max_tick_ops ? p->max_tick_ops : t->submax;
+    sv.max_tick_reqs = p->max_tick_reqs ? p->max_tick_reqs : 0xffffffffu;
+    OpsDev o = make_ops(ops, 0, keys, t->key_bytes, t->is_map ? values : nullptr, t->val_bytes);
+    OutDev d = t->is_map ? make_out(out, nullptr, nullptr)
+                         : make_out(nullptr, nullptr, static_cast<uint8_t*>(out));
+    BarState* bar = &t->state->bar;
+    ServeState* ss = &t->state->serve;
+    const int bt = t->tun.block_threads;
+    rc = dispatch(t, [&](auto l) {
+        using L = decltype(l);
+        return dispatch_g(t->tun.lanes_per_op, [&](auto gc) {
+            constexpr int G = decltype(gc)::value;
+            int per_cu = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_serve<L, G>, bt, 0) !=
+                    hipSuccess ||
+                per_cu <= 0)
+                return LSLAB_ERR_HIP;
+            uint64_t grid = (uint64_t)per_cu * (uint64_t)t->num_cus;
+            if (t->tun.max_blocks > 0) grid = std::min<uint64_t>(grid, (uint64_t)t->tun.max_blocks);
+            hipLaunchKernelGGL(k_reset_serve, dim3(1), dim3(64), 0, stream, ss, bar);
+            hipLaunchKernelGGL((k_serve<L, G>), dim3((unsigned)grid), dim3(bt), 0, stream, t->dev, o,
+                               d, sv, ss, bar, t->submax);
+            return hipGetLastError() == hipSuccess ? LSLAB_OK : LSLAB_ERR_HIP;
+        });
+    });
+    if (rc) return rc;
+    return end_call(t, stream);
+}
+
+extern "C" int lslab_serve_result(lslab_table* t, lslab_serve_stats* out, hipStream_t stream) {
+    if (!t || !out) return LSLAB_ERR_INVALID;
+    HIPCHK(hipStreamWaitEvent(stream, t->order_ev, 0));
+    HIPCHK(hipStreamSynchronize(stream));
+    ServeState ss;
+    HIPCHK(hipMemcpy(&ss, &t->state->serve, sizeof(ss), hipMemcpyDeviceToHost));
+    out->ticks = ss.ticks;
+    out->rejected = ss.rejected;
+    out->served_ops = ss.served_ops;
+    out->elapsed_ticks = ss.end;
+    return LSLAB_OK;
 }
 
 extern "C" int lslab_status(lslab_table* t, hipStream_t stream) {

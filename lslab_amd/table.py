@@ -180,6 +180,39 @@ class Map(_Table):
                                                   _stream(stream)), "exec_batches")
         return self.split_pairs(out)
 
+    def serve(self, req_offsets, arrival_ticks, ops, keys, values, req_reserve=None,
+              max_tick_ops=0, max_tick_reqs=0, stream=None):
+        """Device-resident request loop (lslab_serve).  Returns a dict with the
+        per-op results, per-request completion ticks and tick ids, and the
+        loop's counters."""
+        n = keys.numel()
+        nreq = arrival_ticks.numel()
+        _ops(ops, n)
+        _dev(keys, self.kb, "keys")
+        _dev(values, self.vb, "values")
+        _dev(req_offsets, 8, "req_offsets")
+        _dev(arrival_ticks, 8, "arrival_ticks")
+        if req_reserve is not None:
+            _dev(req_reserve, 4, "req_reserve")
+        if req_offsets.numel() != nreq + 1:
+            raise ValueError("req_offsets must have nreq + 1 entries")
+        out = self._pairs(n)
+        complete = torch.zeros(nreq, dtype=torch.int64, device="cuda")
+        tick_of = torch.zeros(nreq, dtype=torch.int32, device="cuda")
+        p = _lib.ServeParams(nreq, req_offsets.data_ptr(), arrival_ticks.data_ptr(),
+                             req_reserve.data_ptr() if req_reserve is not None else None,
+                             complete.data_ptr(), tick_of.data_ptr(), max_tick_ops, max_tick_reqs)
+        lib = _lib.load()
+        _lib.check(lib.lslab_serve(self._h, ctypes.byref(p), _ptr(ops), _ptr(keys), _ptr(values),
+                                   _ptr(out), _stream(stream)), "serve")
+        st = _lib.ServeStats()
+        _lib.check(lib.lslab_serve_result(self._h, ctypes.byref(st), _stream(stream)),
+                   "serve_result")
+        f, v = self.split_pairs(out)
+        return {"found": f, "value": v, "complete_ticks": complete, "tick_of": tick_of,
+                "ticks": st.ticks, "rejected": st.rejected, "served_ops": st.served_ops,
+                "elapsed_ticks": st.elapsed_ticks}
+
     def dump(self, stream=None):
         """All present (key, value) pairs, sorted by key (as unsigned)."""
         st = self.stats(stream)

@@ -162,6 +162,10 @@ def test_phase_separated_bit_exact(lslab, oracle_mod, is_map, kb, vb, engine, la
         f, v = run_batch(tbl, is_map, kb, vb, ops, keys, vals)
         bad = np.flatnonzero((f != f_ref) | (v != v_ref))
         assert len(bad) == 0, (bi, bad[:5], f[bad[:5]], f_ref[bad[:5]], v[bad[:5]], v_ref[bad[:5]])
+        if bi < 5:
+            # Until a batch mixes removes with inserts of other keys in one
+            # bucket, slot placement is order-free, so pool use must match.
+            assert tbl.stats()["slabs_used"] == ref.slabs_used, bi
     assert tbl.status() == 0
     k, vv = dump_sorted(tbl, is_map, kb, vb)
     rk, rv = oracle_dump_sorted(ref)
@@ -169,10 +173,6 @@ def test_phase_separated_bit_exact(lslab, oracle_mod, is_map, kb, vb, engine, la
     np.testing.assert_array_equal(vv, rv)
     st = tbl.stats()
     assert st["present"] == ref.present
-    if engine == 0:
-        # The plain engine applies a phase-separated batch's inserts with the
-        # reference's placement rule, so pool use matches exactly.
-        assert st["slabs_used"] == ref.slabs_used
     assert st["max_chain"] >= 2
 
 

@@ -1,0 +1,137 @@
+"""Served form (device-resident request loop with continuous batching).
+
+- Each request's results are the same alone and in the mix: requests with
+  disjoint keys (distinct within a request) are served together, under
+  arrival-time admission, and every result must equal the oracle's replay of
+  that request alone on the same pre-state (bit-exact; the tolerance is 0).
+- Requests sharing hot keys: the ticks the loop formed are batches, checked
+  in order with the per-key serial-order checker.
+- Admission by pool headroom: a request whose reservation cannot fit is
+  rejected and none of its ops touch the table.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle.oracle import Replay
+
+pytestmark = pytest.mark.gpu
+
+TICKS_PER_US = 100  # the device's constant clock runs at 100 MHz
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+def _t(a, dt=np.int64):
+    return torch.from_numpy(np.ascontiguousarray(a).astype(dt)).cuda()
+
+
+def test_alone_equals_mix(lslab, oracle_mod):
+    say("serve alone == mix")
+    rng = np.random.default_rng(3)
+    nreq = 300
+    sizes = rng.integers(1, 600, size=nreq)
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    n = int(offs[-1])
+    # request r uses keys r * 2^20 + distinct small offsets
+    keys = np.concatenate([np.uint64(r << 20) + rng.permutation(1 << 12)[:s].astype(np.uint64)
+                           for r, s in enumerate(sizes)])
+    ops = rng.choice([0, 1, 2], size=n, p=[0.5, 0.4, 0.1]).astype(np.uint8)
+    vals = rng.integers(1, 2**63, size=n, dtype=np.uint64)
+    # pre-state: a third of each request's candidate keys present
+    pre_k = np.concatenate([np.uint64(r << 20) + np.arange(0, 1 << 12, 3, dtype=np.uint64)
+                            for r in range(nreq)])
+    pre_v = pre_k * np.uint64(7) + np.uint64(1)
+    m = lslab.Map(8, 8, 14, 1 << 16, lslab.tuning(engine=1, max_subbatch=1 << 14))
+    m.put(_t(pre_k.view(np.int64)), _t(pre_v.view(np.int64)))
+    # bursts of simultaneous arrivals separated by gaps, ~3 ms in total
+    arrival = np.sort(rng.integers(0, 3000, size=nreq)) * TICKS_PER_US // 10 * 10
+    res = m.serve(_t(offs), _t(arrival), _t(ops, np.uint8), _t(keys.view(np.int64)),
+                  _t(vals.view(np.int64)), max_tick_ops=1 << 14)
+    assert m.status() == 0
+    assert res["rejected"] == 0 and res["served_ops"] == n
+    tick_of = res["tick_of"].cpu().numpy()
+    assert res["ticks"] >= 2 and len(np.unique(tick_of)) == res["ticks"]
+    assert (np.bincount(tick_of) > 1).any(), "no tick served two requests together"
+    comp = res["complete_ticks"].cpu().numpy()
+    assert (comp >= arrival).all()
+    f = res["found"].cpu().numpy()
+    v = res["value"].contiguous().cpu().numpy().view(np.uint64)
+    for r in range(nreq):
+        lo, hi = offs[r], offs[r + 1]
+        alone = Replay(True, 8, 8, 14, pool_slabs=1 << 16)
+        mine = (pre_k >> np.uint64(20)) == np.uint64(r)
+        alone.apply(np.full(int(mine.sum()), 1, np.uint8), pre_k[mine], pre_v[mine])
+        rf, rv, _ = alone.apply(ops[lo:hi], keys[lo:hi], vals[lo:hi])
+        np.testing.assert_array_equal(f[lo:hi], rf, err_msg=f"request {r}")
+        np.testing.assert_array_equal(v[lo:hi], rv, err_msg=f"request {r}")
+
+
+def test_shared_hot_keys_serial_order(lslab, oracle_mod):
+    say("serve hot keys")
+    rng = np.random.default_rng(9)
+    nreq = 200
+    sizes = rng.integers(1, 400, size=nreq)
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    n = int(offs[-1])
+    space = 2000
+    p = 1.0 / np.arange(1, space + 1) ** 1.2
+    p /= p.sum()
+    keys = rng.choice(space, size=n, p=p).astype(np.uint64) * np.uint64(0x9E3779B97F4A7C15)
+    ops = rng.choice([0, 1], size=n).astype(np.uint8)
+    vals = rng.integers(1, 2**63, size=n, dtype=np.uint64)
+    m = lslab.Map(8, 8, 10, 1 << 14, lslab.tuning(engine=1, max_subbatch=1 << 13))
+    arrival = np.sort(rng.integers(0, 2000 * TICKS_PER_US, size=nreq))
+    res = m.serve(_t(offs), _t(arrival), _t(ops, np.uint8), _t(keys.view(np.int64)),
+                  _t(vals.view(np.int64)), max_tick_ops=1 << 13)
+    assert m.status() == 0 and res["rejected"] == 0
+    tick_of = res["tick_of"].cpu().numpy()
+    assert (np.diff(tick_of) >= 0).all()  # admitted in arrival order
+    f = res["found"].cpu().numpy()
+    v = res["value"].contiguous().cpu().numpy().view(np.uint64)
+    ref = Replay(True, 8, 8, 10, pool_slabs=1 << 14)
+    for tk in range(int(res["ticks"])):
+        rs = np.flatnonzero(tick_of == tk)
+        lo, hi = offs[rs[0]], offs[rs[-1] + 1]
+        bad, first = ref.check_batch(ops[lo:hi], keys[lo:hi], vals[lo:hi], f[lo:hi], v[lo:hi])
+        assert bad == 0, (tk, first)
+    k, vv = m.dump()
+    k = k.cpu().numpy().view(np.uint64)
+    vv = vv.cpu().numpy().view(np.uint64)
+    rk, rv = ref.dump()
+    o1, o2 = np.argsort(k), np.argsort(rk)
+    np.testing.assert_array_equal(k[o1], rk[o2])
+    np.testing.assert_array_equal(vv[o1], rv[o2])
+
+
+def test_admission_by_headroom(lslab, oracle_mod):
+    say("serve admission")
+    # One bucket and a 40-slab pool; a request reserves one slab per put and
+    # is admitted only while slabs used + its reservation <= 40.
+    sizes = np.array([31, 10, 35, 50, 5])
+    nreq = len(sizes)
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    n = int(offs[-1])
+    keys = np.arange(n, dtype=np.uint64)  # all distinct: every put inserts
+    ops = np.ones(n, np.uint8)
+    vals = keys + np.uint64(1000)
+    m = lslab.Map(8, 8, 0, 40, lslab.tuning(engine=1, max_subbatch=1 << 12))
+    arrival = np.arange(nreq, dtype=np.int64) * 50 * TICKS_PER_US
+    res = m.serve(_t(offs), _t(arrival), _t(ops, np.uint8), _t(keys.view(np.int64)),
+                  _t(vals.view(np.int64)), req_reserve=_t(sizes, np.int32), max_tick_ops=64,
+                  max_tick_reqs=1)
+    assert m.status() == 0
+    tick_of = res["tick_of"].cpu().numpy().view(np.uint32)
+    # r0 fills the head slab (0 used), r1 takes 1 slab, r2 one more (2 used):
+    # r3 would need 2 + 50 > 40 slabs and is refused; r4 fits.
+    assert (tick_of == 0xFFFFFFFF).tolist() == [False, False, False, True, False]
+    assert res["rejected"] == 1 and res["ticks"] == 4
+    present = np.concatenate([keys[offs[r]:offs[r + 1]] for r in (0, 1, 2, 4)])
+    k, v = m.dump()
+    k = k.cpu().numpy().view(np.uint64)
+    assert sorted(k.tolist()) == sorted(present.tolist())
+    ref = Replay(True, 8, 8, 0, pool_slabs=40)
+    ref.apply(np.ones(len(present), np.uint8), present, present + np.uint64(1000))
+    assert m.stats()["slabs_used"] == ref.slabs_used == 2
