@@ -43,7 +43,7 @@ __device__ __forceinline__ void load_op(const OpsDev& o, uint64_t i, uint8_t& op
 //   get    -> (present, value)      put -> (was present, old value)
 //   remove -> (was present, value)  set insert -> (true, 0)
 // and ok=false if the pool was exhausted.
-template <class L, int G>
+template <class L, int G, bool COH>
 __device__ __forceinline__ void apply_op(const TableDev& t, uint8_t op, typename L::K key,
                                          typename L::V val, uint64_t b, bool& rfound,
                                          typename L::V& rval, bool& ok) {
@@ -52,11 +52,11 @@ __device__ __forceinline__ void apply_op(const TableDev& t, uint8_t op, typename
     uint64_t s;
     int slot;
     uint32_t valid;
-    walk<L, G>(t, b, key, s, slot, valid);
+    walk<L, G, COH>(t, b, key, s, slot, valid);
     const bool found = slot >= 0;
     V cur = V(0);
     if constexpr (L::IS_MAP) {
-        if (found) cur = *val_slot<L>(t, s, slot);
+        if (found) cur = COH ? ld_coh(val_slot<L>(t, s, slot)) : *val_slot<L>(t, s, slot);
     }
     ok = true;
     rfound = found;
@@ -67,7 +67,7 @@ __device__ __forceinline__ void apply_op(const TableDev& t, uint8_t op, typename
     if (op == OP_PUT) {
         if (found) {
             if constexpr (L::IS_MAP) {
-                if (gl == 0) *val_slot<L>(t, s, slot) = val;
+                if (gl == 0) st_coh(val_slot<L>(t, s, slot), val);
             }
         } else {
             ins = true;
@@ -108,11 +108,17 @@ __device__ __forceinline__ void apply_op(const TableDev& t, uint8_t op, typename
         }
     }
     if (ins && gl == 0) {
-        *key_slot<L>(t, ins_s, ins_slot) = key;
-        if constexpr (L::IS_MAP) *val_slot<L>(t, ins_s, ins_slot) = val;
+        // Key and value first, then the valid bit, then the link: each step
+        // completes before the next is issued.
+        st_coh(key_slot<L>(t, ins_s, ins_slot), key);
+        if constexpr (L::IS_MAP) st_coh(val_slot<L>(t, ins_s, ins_slot), val);
+        wait_vm();
         __hip_atomic_fetch_or(valid_word<L>(t, ins_s), 1u << ins_slot, __ATOMIC_RELAXED,
                               __HIP_MEMORY_SCOPE_AGENT);
-        if (ins_s != s) *next_word<L>(t, s) = (uint32_t)ins_s;  // link after filling
+        if (ins_s != s) {
+            wait_vm();
+            st_coh(next_word<L>(t, s), (uint32_t)ins_s);
+        }
     }
 }
 
@@ -124,7 +130,7 @@ __device__ __forceinline__ void run_op(const TableDev& t, uint8_t op, typename L
                                        bool& ok) {
     const uint64_t b = (uint64_t)key & t.mask;  // identity hash, masked (hash.h:10-15)
     if constexpr (!LOCKED) {
-        apply_op<L, G>(t, op, key, val, b, rfound, rval, ok);
+        apply_op<L, G, false>(t, op, key, val, b, rfound, rval, ok);
         return;
     } else {
         const int gl = Grp<G>::lane();
@@ -132,35 +138,42 @@ __device__ __forceinline__ void run_op(const TableDev& t, uint8_t op, typename L
         uint32_t* lw = t.locks + b;
         bool done = false;
         uint64_t t0 = 0;
-        while (!done) {
-            int got = 0;
-            if (gl == 0) got = shared ? try_lock_shared(lw) : try_lock_excl(lw);
-            got = __shfl(got, 0, G);
-            if (got) {
-                acquire_agent();
-                apply_op<L, G>(t, op, key, val, b, rfound, rval, ok);
-                if (shared) {
-                    // The slab reads above must complete before the lock drops.
-                    __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
-                    if (gl == 0) unlock_shared(lw);
-                } else {
-                    release_agent();
-                    if (gl == 0) unlock_excl(lw);
-                }
-                done = true;
-            } else {
-                uint64_t now = now_ticks();
-                if (t0 == 0) t0 = now;
-                if (now - t0 > SPIN_LIMIT_TICKS) {
-                    if (gl == 0) __hip_atomic_fetch_or(t.status, ST_TIMEOUT, __ATOMIC_RELAXED,
-                                                       __HIP_MEMORY_SCOPE_AGENT);
-                    rfound = false;
-                    rval = typename L::V(0);
-                    ok = false;
+        // The loop exits only when every lane of the wave is done (a
+        // wave-uniform branch).  With a divergent exit the compiler may run the
+        // "got the lock" block -- critical section and unlock -- on the loop's
+        // exit path, i.e. after all lanes have left the loop, so a group of the
+        // same wave spinning on that lock would never see it released.
+        for (;;) {
+            if (!done) {
+                int got = 0;
+                if (gl == 0) got = shared ? try_lock_shared(lw) : try_lock_excl(lw);
+                got = __shfl(got, 0, G);
+                if (got) {
+                    // Inside the lock every slab access is agent-coherent, so
+                    // the lock needs no cache-wide fence: the critical section's
+                    // reads and writes complete (wait_vm) before the word drops.
+                    apply_op<L, G, true>(t, op, key, val, b, rfound, rval, ok);
+                    wait_vm();
+                    if (gl == 0) {
+                        if (shared) unlock_shared(lw);
+                        else unlock_excl(lw);
+                    }
                     done = true;
+                } else {
+                    const uint64_t now = now_ticks();
+                    if (t0 == 0) t0 = now;
+                    if (now - t0 > SPIN_LIMIT_TICKS) {
+                        if (gl == 0) __hip_atomic_fetch_or(t.status, ST_TIMEOUT, __ATOMIC_RELAXED,
+                                                           __HIP_MEMORY_SCOPE_AGENT);
+                        rfound = false;
+                        rval = typename L::V(0);
+                        ok = false;
+                        done = true;
+                    }
                 }
-                __builtin_amdgcn_s_sleep(1);
             }
+            if (__ballot(!done) == 0) break;
+            __builtin_amdgcn_s_sleep(1);
         }
     }
 }
@@ -200,7 +213,7 @@ __device__ __forceinline__ void phase_r_op(const TableDev& t, const OpsDev& o, c
     uint64_t s;
     int slot;
     uint32_t valid;
-    walk<L, G>(t, (uint64_t)key & t.mask, key, s, slot, valid);
+    walk<L, G, false>(t, (uint64_t)key & t.mask, key, s, slot, valid);
     const bool found = slot >= 0;
     V cur = V(0);
     if constexpr (L::IS_MAP) {
@@ -272,7 +285,7 @@ __device__ __forceinline__ void phase_w_op(const TableDev& t, const OpsDev& o, u
         const int slot = (int)(loc & 31u);
         if (gl == 0) {
             if (op == OP_PUT) {
-                if constexpr (L::IS_MAP) *val_slot<L>(t, s, slot) = val;
+                if constexpr (L::IS_MAP) st_coh(val_slot<L>(t, s, slot), val);
             } else {
                 __hip_atomic_fetch_and(valid_word<L>(t, s), ~(1u << slot), __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);

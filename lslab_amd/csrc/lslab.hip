@@ -57,12 +57,15 @@ __global__ __launch_bounds__(256) void k_phase_w(TableDev t, OpsDev o, uint64_t 
     for (uint64_t li = gid; li < m; li += ng) phase_w_op<L, G>(t, o, lo, (uint32_t)li);
 }
 
-// Grid barrier for a grid sized to full residency.  Every wave publishes its
-// writes (agent release), one lane per block arrives, the last arrival bumps
-// the generation, everyone acquires.  Spins are bounded; on timeout or abort
+// Grid barrier for a grid sized to full residency.  Every wave completes its
+// writes, one lane per block arrives, the last arrival bumps the generation,
+// every wave acquires.  Spins are bounded; on timeout or abort
 // the barrier returns false and the caller leaves the kernel.
 __device__ __forceinline__ bool grid_sync(BarState* bar, uint32_t nblocks) {
-    release_agent();
+    // Shared table writes are agent-scope stores/atomics, so arrival only
+    // needs them complete; the acquire below drops stale cached lines before
+    // the next phase reads the table with plain loads.
+    wait_vm();
     __syncthreads();
     __shared__ uint32_t ok_s;
     if (threadIdx.x == 0) {
@@ -199,11 +202,12 @@ __device__ void serve_schedule(const TableDev& t, const ServeDev& sv, ServeState
         n++;
         r++;
     }
-    ss->r_lo = lo;
-    ss->r_hi = r;
+    // Read by every block after the barrier: agent-scope stores.
+    st_coh(&ss->r_lo, lo);
+    st_coh(&ss->r_hi, r);
     ss->next_req = r;
     ss->served_ops += ops;
-    if (n == 0) ss->done = 1;
+    if (n == 0) st_coh(&ss->done, 1u);
 }
 
 template <class L, int G>
@@ -218,7 +222,7 @@ __global__ __launch_bounds__(256) void k_serve(TableDev t, OpsDev o, OutDev out,
         if (!grid_sync(bar, gridDim.x)) goto fail;
         {
             if (ld_agent(&ss->done)) break;
-            const uint64_t r_lo = ss->r_lo, r_hi = ss->r_hi;
+            const uint64_t r_lo = ld_coh(&ss->r_lo), r_hi = ld_coh(&ss->r_hi);
             const uint64_t lo = sv.off[r_lo], hi = sv.off[r_hi];
             for (uint64_t slo = lo; slo < hi; slo += submax) {
                 const uint64_t m = min(submax, hi - slo);

@@ -134,6 +134,40 @@ __device__ __forceinline__ void load_slots(const K* p, K (&out)[S]) {
     }
 }
 
+// Agent-coherent variant: slab words that other waves of the same launch may
+// be writing are read with agent-scope atomic loads (global_load ... sc1), so
+// no stale cache line can answer and no cache-wide acquire fence is needed.
+template <typename K, int S>
+__device__ __forceinline__ void load_slots_coherent(const K* p, K (&out)[S]) {
+    constexpr int BYTES = S * (int)sizeof(K);
+    if constexpr (BYTES >= 8) {
+        const uint64_t* q = reinterpret_cast<const uint64_t*>(p);
+#pragma unroll
+        for (int j = 0; j < BYTES / 8; ++j) {
+            uint64_t w = __hip_atomic_load(q + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __builtin_memcpy(reinterpret_cast<char*>(out) + 8 * j, &w, 8);
+        }
+    } else {
+        uint32_t w = __hip_atomic_load(reinterpret_cast<const uint32_t*>(p), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_memcpy(out, &w, 4);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T ld_coh(const T* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Write-through store of a shared slab word (global_store ... sc1).
+template <typename T>
+__device__ __forceinline__ void st_coh(T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Waits until this wave's earlier memory operations have completed; with
+// every shared write issued as an agent-scope store or atomic, this is all a
+// lock release or barrier arrival needs (no cache-wide write-back).
+__device__ __forceinline__ void wait_vm() { __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
 template <class L>
 __device__ __forceinline__ char* slab_ptr(const TableDev& t, uint64_t s) {
     return t.slabs + s * (uint64_t)L::SLAB_BYTES;
@@ -165,16 +199,18 @@ struct Probe {
 
 // Group-cooperative probe of slab s for key.  Lane l of the group loads key
 // slots [l*S, l*S+S) in one coalesced access (a u64 key block is two 128-B
-// lines read by the whole group at once).  Plain loads: callers guarantee no
-// concurrent writer, or hold the bucket lock after an acquire.
-template <class L, int G>
+// lines read by the whole group at once).  Plain loads when the caller
+// guarantees no concurrent writer (read-only launches, phase R); coherent
+// loads (COH) under a bucket lock.
+template <class L, int G, bool COH>
 __device__ __forceinline__ Probe probe_slab(const TableDev& t, uint64_t s, typename L::K key) {
     using K = typename L::K;
     constexpr int S = Grp<G>::S;
     const int gl = Grp<G>::lane();
     const K* kb = reinterpret_cast<const K*>(slab_ptr<L>(t, s));
     K mine[S];
-    load_slots<K, S>(kb + gl * S, mine);
+    if constexpr (COH) load_slots_coherent<K, S>(kb + gl * S, mine);
+    else load_slots<K, S>(kb + gl * S, mine);
     Probe p;
     if constexpr (sizeof(K) == 8) {
         uint64_t meta = grp_bcast((uint64_t)mine[S - 1], G - 1, G);
@@ -199,14 +235,16 @@ __device__ __forceinline__ Probe probe_slab(const TableDev& t, uint64_t s, typen
         p.slot = grp_bcast(hit, src, G);
     } else {
         p.slot = -1;
-        if constexpr (sizeof(K) == 4 && L::IS_MAP) p.next = *next_word<L>(t, s);
+        if constexpr (sizeof(K) == 4 && L::IS_MAP)
+            p.next = COH ? ld_coh(next_word<L>(t, s)) : *next_word<L>(t, s);
     }
     return p;
 }
 
 // Walks the chain of bucket b: returns the slab/slot holding key, or slot -1
-// with `s` = last slab of the chain and `valid` its bitmap.
-template <class L, int G>
+// with `s` = last slab of the chain and `valid` its bitmap.  COH selects the
+// agent-coherent loads (inside locked sections, where other waves write).
+template <class L, int G, bool COH>
 __device__ __forceinline__ void walk(const TableDev& t, uint64_t b, typename L::K key,
                                      uint64_t& s, int& slot, uint32_t& valid) {
     s = b;
@@ -214,7 +252,7 @@ __device__ __forceinline__ void walk(const TableDev& t, uint64_t b, typename L::
     // array; both are checked so that a damaged chain ends the walk with a
     // status bit instead of a runaway loop or a wild access.
     for (uint64_t hops = 0;; ++hops) {
-        Probe p = probe_slab<L, G>(t, s, key);
+        Probe p = probe_slab<L, G, COH>(t, s, key);
         if (p.slot >= 0) {
             slot = p.slot;
             valid = p.valid;
