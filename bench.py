@@ -272,6 +272,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--served-requests", type=int, default=20000)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run a 1 GiB device copy first, as the HBM byte-counter reference")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -284,6 +286,12 @@ def main():
 
     torch.cuda.set_device(0)
     K, W = args.steps, args.warmup
+    if args.copy_calibration:
+        src = torch.ones(1 << 27, dtype=torch.int64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     tun = lslab_amd.tuning(lanes_per_op=args.lanes, engine=lslab_amd.ENGINE_PLAIN,
                            max_subbatch=BATCH)
     m = lslab_amd.Map(8, 8, N_LOG2, 1 << 16, tun)

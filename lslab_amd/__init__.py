@@ -14,11 +14,10 @@ __all__ = ["Map", "Set", "tuning", "workload", "ENGINE_PLAIN", "ENGINE_TWO_PHASE
 
 def __getattr__(name):
     # torch-dependent parts load lazily so that the C ABI can be inspected
-    # without initialising a device.
+    # without importing torch.
+    import importlib
     if name in ("Map", "Set", "tuning"):
-        from . import table
-        return getattr(table, name)
-    if name == "workload":
-        from . import workload
-        return workload
+        return getattr(importlib.import_module(__name__ + ".table"), name)
+    if name in ("workload", "table", "autotune", "build"):
+        return importlib.import_module(__name__ + "." + name)
     raise AttributeError(name)

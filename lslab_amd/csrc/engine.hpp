@@ -197,11 +197,13 @@ __device__ __forceinline__ uint32_t gt_find_or_insert(const TableDev& t, uint64_
     return 0xffffffffu;  // unreachable while a sub-batch holds <= capacity/2 ops
 }
 
-// Phase R of op (lo + li): lock-free lookup of the pre-batch state, result of
-// gets, and the per-key exchange for mutations.
+// Phase R of op (lo + li) of a sub-batch of m ops: lock-free lookup of the
+// pre-batch state, result of gets, and the per-key exchange for mutations.
+// The exchange order is the order in which waves reach the atomic, not op
+// index order, so a predecessor may have any index of the sub-batch.
 template <class L, int G>
 __device__ __forceinline__ void phase_r_op(const TableDev& t, const OpsDev& o, const OutDev& out,
-                                           uint64_t lo, uint32_t li) {
+                                           uint64_t lo, uint32_t li, uint64_t m) {
     using K = typename L::K;
     using V = typename L::V;
     const int gl = Grp<G>::lane();
@@ -234,7 +236,7 @@ __device__ __forceinline__ void phase_r_op(const TableDev& t, const OpsDev& o, c
         }
         bool rf = found;
         V rv = cur;
-        if (pred > li) {  // entries only ever hold earlier ops of this sub-batch
+        if (pred > m) {  // entries only ever hold ops of this sub-batch (any order)
             __hip_atomic_fetch_or(t.status, ST_CORRUPT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             pred = 0;
         }

@@ -162,9 +162,12 @@ def test_phase_separated_bit_exact(lslab, oracle_mod, is_map, kb, vb, engine, la
         f, v = run_batch(tbl, is_map, kb, vb, ops, keys, vals)
         bad = np.flatnonzero((f != f_ref) | (v != v_ref))
         assert len(bad) == 0, (bi, bad[:5], f[bad[:5]], f_ref[bad[:5]], v[bad[:5]], v_ref[bad[:5]])
-        if bi < 5:
-            # Until a batch mixes removes with inserts of other keys in one
-            # bucket, slot placement is order-free, so pool use must match.
+        if bi < 4:
+            # Which slot a key gets depends on the order of concurrent inserts,
+            # so after removes (batch 2) the holes left in each chain's last
+            # slab -- and the slabs later inserts need -- may differ from the
+            # sequential replay.  Until an insert follows a remove, pool use is
+            # order-free and must match exactly.
             assert tbl.stats()["slabs_used"] == ref.slabs_used, bi
     assert tbl.status() == 0
     k, vv = dump_sorted(tbl, is_map, kb, vb)
