@@ -709,6 +709,25 @@ extern "C" int lslab_set_mixed(lslab_table* t, const uint8_t* ops, const void* k
     return run_ops(t, o, make_out(nullptr, nullptr, out), n, t->tun.engine, false, stream);
 }
 
+// Grid of a persistent kernel whose blocks wait on each other at grid
+// barriers: every block must be resident at once.  The occupancy query counts
+// registers per block but not the SGPR file (a kernel at 64 VGPRs and 106
+// SGPRs fits 7 waves per SIMD, not the 8 its VGPRs allow), so the grid is
+// held to at most 4 blocks of <= 256 threads per CU -- 16 waves per CU, which
+// already saturates random slab reads -- and never above the query.
+static int persistent_grid(const lslab_table* t, const void* kernel, int bt, uint64_t* grid) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, bt, 0) != hipSuccess ||
+        per_cu <= 0)
+        return LSLAB_ERR_HIP;
+    const int cap = std::max(1, 1024 / bt);  // <= 16 waves per CU
+    per_cu = std::min(per_cu - (per_cu > 1 ? 1 : 0), cap);
+    uint64_t g = (uint64_t)per_cu * (uint64_t)t->num_cus;
+    if (t->tun.max_blocks > 0) g = std::min<uint64_t>(g, (uint64_t)t->tun.max_blocks);
+    *grid = g;
+    return LSLAB_OK;
+}
+
 extern "C" int lslab_exec_batches(lslab_table* t, size_t nbatches, const uint64_t* offsets,
                                   const uint8_t* ops, const void* keys, const void* values,
                                   void* out, hipStream_t stream) {
@@ -726,12 +745,9 @@ extern "C" int lslab_exec_batches(lslab_table* t, size_t nbatches, const uint64_
         using L = decltype(l);
         return dispatch_g(t->tun.lanes_per_op, [&](auto gc) {
             constexpr int G = decltype(gc)::value;
-            int per_cu = 0;
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batches<L, G>, bt, 0) !=
-                    hipSuccess ||
-                per_cu <= 0)
-                return LSLAB_ERR_HIP;
-            uint64_t grid = (uint64_t)per_cu * (uint64_t)t->num_cus;
+            uint64_t grid = 0;
+            const int rc2 = persistent_grid(t, reinterpret_cast<const void*>(k_batches<L, G>), bt, &grid);
+            if (rc2) return rc2;
             if (t->tun.max_blocks > 0) grid = std::min<uint64_t>(grid, (uint64_t)t->tun.max_blocks);
             hipLaunchKernelGGL(k_reset_bar, dim3(1), dim3(64), 0, stream, bar);
             hipLaunchKernelGGL((k_batches<L, G>), dim3((unsigned)grid), dim3(bt), 0, stream, t->dev,
@@ -770,12 +786,9 @@ extern "C" int lslab_serve(lslab_table* t, const lslab_serve_params* p, const ui
         using L = decltype(l);
         return dispatch_g(t->tun.lanes_per_op, [&](auto gc) {
             constexpr int G = decltype(gc)::value;
-            int per_cu = 0;
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_serve<L, G>, bt, 0) !=
-                    hipSuccess ||
-                per_cu <= 0)
-                return LSLAB_ERR_HIP;
-            uint64_t grid = (uint64_t)per_cu * (uint64_t)t->num_cus;
+            uint64_t grid = 0;
+            const int rc2 = persistent_grid(t, reinterpret_cast<const void*>(k_serve<L, G>), bt, &grid);
+            if (rc2) return rc2;
             if (t->tun.max_blocks > 0) grid = std::min<uint64_t>(grid, (uint64_t)t->tun.max_blocks);
             hipLaunchKernelGGL(k_reset_serve, dim3(1), dim3(64), 0, stream, ss, bar);
             hipLaunchKernelGGL((k_serve<L, G>), dim3((unsigned)grid), dim3(bt), 0, stream, t->dev, o,
