@@ -138,6 +138,7 @@ __device__ __forceinline__ void run_op(const TableDev& t, uint8_t op, typename L
         uint32_t* lw = t.locks + b;
         bool done = false;
         uint64_t t0 = 0;
+        uint32_t backoff = 1;
         // The loop exits only when every lane of the wave is done (a
         // wave-uniform branch).  With a divergent exit the compiler may run the
         // "got the lock" block -- critical section and unlock -- on the loop's
@@ -173,7 +174,9 @@ __device__ __forceinline__ void run_op(const TableDev& t, uint8_t op, typename L
                 }
             }
             if (__ballot(!done) == 0) break;
-            __builtin_amdgcn_s_sleep(1);
+            // exponential backoff of the whole wave while a group still waits
+            for (uint32_t i = 0; i < backoff; ++i) __builtin_amdgcn_s_sleep(4);
+            backoff = backoff < 64 ? backoff * 2 : 64;
         }
     }
 }
@@ -225,15 +228,43 @@ __device__ __forceinline__ void phase_r_op(const TableDev& t, const OpsDev& o, c
         if (gl == 0) write_result<L>(out, i, found, cur);
         return;
     }
+    // Wave-level combining: the mutation leaders of this wave that share a
+    // key are chained in lane order, and only the last of them touches the
+    // key's combining entry (one lookup, one exchange per key per wave instead
+    // of per op -- a Zipf-hot key's exchanges are what serialise).
+    const int lane = (int)__lane_id();
+    const bool lead = (gl == 0);
+    const uint64_t kk = (uint64_t)key;
+    int my_last = lane, my_prev = -1;
+    uint64_t todo = __ballot(lead);
+    while (todo) {  // wave-uniform: one pass per distinct key among the leaders
+        const int l = __builtin_ctzll(todo);
+        const uint64_t kl = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(kk >> 32), l) << 32) |
+                            (uint32_t)__builtin_amdgcn_readlane((uint32_t)kk, l);
+        const bool mine = lead && kk == kl;
+        const uint64_t same = __ballot(mine);
+        todo &= ~same;
+        if (mine) {
+            my_last = 63 - __builtin_clzll(same);
+            const uint64_t below = same & ((1ull << lane) - 1ull);
+            my_prev = below ? 63 - __builtin_clzll(below) : -1;
+        }
+    }
     uint32_t g = 0, pred = 0;
-    if (gl == 0) {
-        g = gt_find_or_insert(t, (uint64_t)key);
+    if (lead && lane == my_last) {
+        g = gt_find_or_insert(t, kk);
         if (g != 0xffffffffu) {
             pred = __hip_atomic_exchange(&t.gt_head[g], li + 1u, __ATOMIC_RELAXED,
                                          __HIP_MEMORY_SCOPE_AGENT);
         } else {
             __hip_atomic_fetch_or(t.status, ST_CORRUPT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
+    }
+    g = __shfl(g, my_last, 64);
+    pred = __shfl(pred, my_last, 64);
+    const uint32_t prev_li = __shfl(li, my_prev >= 0 ? my_prev : lane, 64);
+    if (my_prev >= 0) pred = prev_li + 1u;
+    if (lead) {
         bool rf = found;
         V rv = cur;
         if (pred > m) {  // entries only ever hold ops of this sub-batch (any order)

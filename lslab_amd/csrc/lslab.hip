@@ -16,11 +16,22 @@ using namespace lslab_dev;
 // ---------------------------------------------------------------------------
 
 // Grid-wide barrier state for the persistent batched kernel.
-struct BarState {
+// Grid-wide barrier state for the persistent kernels.  Blocks arrive on one
+// of BAR_GROUPS group counters (block id mod BAR_GROUPS), the last block of a
+// group arrives on the top counter, and the last group releases every group
+// by bumping the group's generation word.  Each word sits on its own 128-B
+// line, so arrivals and polls of different groups do not share a line.
+constexpr uint32_t BAR_GROUPS = 16;
+struct alignas(128) BarWord {
     uint32_t count;
     uint32_t gen;
+    uint32_t pad[30];
+};
+struct BarState {
+    BarWord top;
+    BarWord group[BAR_GROUPS];
     uint32_t abort;
-    uint32_t pad;
+    uint32_t pad[31];
 };
 
 // Plain per-batch kernel: one G-lane group per op, grid-stride.
@@ -57,35 +68,47 @@ __global__ __launch_bounds__(256) void k_phase_w(TableDev t, OpsDev o, uint64_t 
     for (uint64_t li = gid; li < m; li += ng) phase_w_op<L, G>(t, o, lo, (uint32_t)li);
 }
 
-// Grid barrier for a grid sized to full residency.  Every wave completes its
-// writes, one lane per block arrives, the last arrival bumps the generation,
-// every wave acquires.  Spins are bounded; on timeout or abort
-// the barrier returns false and the caller leaves the kernel.
+// Grid barrier for a grid that is fully resident.  Every wave waits for its
+// own writes to complete (shared table writes are agent-scope stores or
+// atomics, so nothing else is needed to publish them), one lane per block
+// arrives, the last arrival releases everyone, and every wave acquires (drops
+// stale cached lines) before the next phase reads the table with plain
+// loads.  No cache-wide write-back is issued.  Spins are bounded; on timeout
+// or abort the barrier returns false and the caller leaves the kernel.
 __device__ __forceinline__ bool grid_sync(BarState* bar, uint32_t nblocks) {
-    // Shared table writes are agent-scope stores/atomics, so arrival only
-    // needs them complete; the acquire below drops stale cached lines before
-    // the next phase reads the table with plain loads.
     wait_vm();
     __syncthreads();
     __shared__ uint32_t ok_s;
     if (threadIdx.x == 0) {
         uint32_t ok = 1;
-        const uint32_t g = ld_agent(&bar->gen);
+        const uint32_t x = blockIdx.x % BAR_GROUPS;
+        const uint32_t ngroups = nblocks < BAR_GROUPS ? nblocks : BAR_GROUPS;
+        const uint32_t members = (nblocks - x + BAR_GROUPS - 1) / BAR_GROUPS;
+        BarWord* gw = &bar->group[x];
+        const uint32_t g = ld_agent(&gw->gen);
         const uint32_t arrived =
-            __hip_atomic_fetch_add(&bar->count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (arrived == nblocks - 1u) {
-            __hip_atomic_store(&bar->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&bar->gen, g + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            const uint64_t t0 = now_ticks();
-            while (ld_agent(&bar->gen) == g) {
-                if (ld_agent(&bar->abort) || now_ticks() - t0 > SPIN_LIMIT_TICKS) {
-                    __hip_atomic_store(&bar->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    ok = 0;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
+            __hip_atomic_fetch_add(&gw->count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (arrived == members - 1u) {
+            st_coh(&gw->count, 0u);
+            wait_vm();
+            const uint32_t top = __hip_atomic_fetch_add(&bar->top.count, 1u, __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_AGENT);
+            if (top == ngroups - 1u) {
+                st_coh(&bar->top.count, 0u);
+                wait_vm();
+                for (uint32_t y = 0; y < ngroups; ++y)
+                    __hip_atomic_fetch_add(&bar->group[y].gen, 1u, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
             }
+        }
+        const uint64_t t0 = now_ticks();
+        while (ld_agent(&gw->gen) == g) {
+            if (ld_agent(&bar->abort) || now_ticks() - t0 > SPIN_LIMIT_TICKS) {
+                st_coh(&bar->abort, 1u);
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
         }
         ok_s = ok;
     }
@@ -124,12 +147,15 @@ __global__ __launch_bounds__(256) void k_batches(TableDev t, OpsDev o, OutDev ou
     }
 }
 
-__global__ void k_reset_bar(BarState* bar) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        bar->count = 0;
-        bar->abort = 0;
+__device__ __forceinline__ void reset_bar(BarState* bar) {
+    if (threadIdx.x <= BAR_GROUPS && blockIdx.x == 0) {
+        BarWord* w = threadIdx.x == 0 ? &bar->top : &bar->group[threadIdx.x - 1];
+        w->count = 0;
+        if (threadIdx.x == 0) bar->abort = 0;
     }
 }
+
+__global__ void k_reset_bar(BarState* bar) { reset_bar(bar); }
 
 // ---------------------------------------------------------------------------
 // Served form: a device-resident request loop.
@@ -254,9 +280,8 @@ __global__ void k_reset_serve(ServeState* ss, BarState* bar) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         ServeState z{};
         *ss = z;
-        bar->count = 0;
-        bar->abort = 0;
     }
+    reset_bar(bar);
 }
 
 // Introspection: one thread per bucket walks its chain.
@@ -392,7 +417,7 @@ static int dispatch_g(int g, F&& f) {
 }
 
 static void normalise(lslab_tuning& tu) {
-    if (tu.lanes_per_op == 0) tu.lanes_per_op = 16;
+    if (tu.lanes_per_op == 0) tu.lanes_per_op = 4;
     if (tu.block_threads == 0) tu.block_threads = 256;
     if (tu.max_subbatch == 0) tu.max_subbatch = 1ull << 20;
 }

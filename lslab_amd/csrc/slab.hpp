@@ -304,15 +304,27 @@ __device__ __forceinline__ uint64_t ld_agent(const uint64_t* p) {
 }
 
 // Wave lock primitives on a bucket's 32-bit word (one lane calls them).
-// 0 = free, LOCK_WRITER = held exclusively, n = n readers.
+// bit 31 = held exclusively, bit 30 = a writer is waiting, bits 0..29 = number
+// of readers.  Attempts read the word first and only then issue the RMW
+// (test-and-test-and-set), and a writer that fails raises the waiting bit,
+// which turns new readers away until it gets in -- so a hot bucket's stream
+// of readers cannot starve its writers.
+constexpr uint32_t LOCK_WAITING = 0x40000000u;
+
 __device__ __forceinline__ bool try_lock_excl(uint32_t* w) {
-    uint32_t e = 0;
-    return __hip_atomic_compare_exchange_strong(w, &e, LOCK_WRITER, __ATOMIC_RELAXED,
-                                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t v = ld_coh(w);
+    if (v == 0 || v == LOCK_WAITING) {
+        return __hip_atomic_compare_exchange_strong(w, &v, LOCK_WRITER, __ATOMIC_RELAXED,
+                                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (!(v & LOCK_WAITING))
+        __hip_atomic_fetch_or(w, LOCK_WAITING, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return false;
 }
 __device__ __forceinline__ bool try_lock_shared(uint32_t* w) {
-    uint32_t old = __hip_atomic_fetch_add(w, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (old & LOCK_WRITER) {
+    if (ld_coh(w) & (LOCK_WRITER | LOCK_WAITING)) return false;
+    const uint32_t old = __hip_atomic_fetch_add(w, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (old & (LOCK_WRITER | LOCK_WAITING)) {
         __hip_atomic_fetch_add(w, 0xffffffffu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return false;
     }

@@ -42,7 +42,7 @@ def _ops(t, n):
     return t
 
 
-def tuning(lanes_per_op=16, engine=_lib.ENGINE_PLAIN, block_threads=256, max_blocks=0,
+def tuning(lanes_per_op=4, engine=_lib.ENGINE_PLAIN, block_threads=256, max_blocks=0,
            max_subbatch=1 << 20):
     return _lib.Tuning(lanes_per_op, engine, block_threads, max_blocks, max_subbatch)
 
