@@ -57,7 +57,7 @@ extern "C" {
 typedef struct lslab_table lslab_table;
 
 typedef struct {
-    int lanes_per_op;      /* lanes probing one slab per op: 4, 8, 16, 32 (0 = 4) */
+    int lanes_per_op;      /* lanes probing one slab per op: 1, 2, 4, 8, 16, 32 (0 = 4) */
     int engine;            /* LSLAB_ENGINE_* for put/remove/mixed calls (0 = PLAIN) */
     int block_threads;     /* threads per block (0 = 256) */
     int max_blocks;        /* grid cap for the per-batch kernels (0 = auto) */

@@ -16,6 +16,11 @@ using namespace lslab_dev;
 // ---------------------------------------------------------------------------
 
 // Grid-wide barrier state for the persistent batched kernel.
+// Persistent kernels run one block of 1024 threads (16 wavefronts) per CU:
+// 16 waves per CU keep random slab reads at full rate, and one block per CU
+// keeps grid barriers cheap (256 arrivals, one cache invalidate per CU).
+constexpr int PERSIST_THREADS = 1024;
+
 // Grid-wide barrier state for the persistent kernels.  Blocks arrive on one
 // of BAR_GROUPS group counters (block id mod BAR_GROUPS), the last block of a
 // group arrives on the top counter, and the last group releases every group
@@ -112,15 +117,22 @@ __device__ __forceinline__ bool grid_sync(BarState* bar, uint32_t nblocks) {
         }
         ok_s = ok;
     }
+    // One acquire per block: wave 0 invalidates the CU's L1 and the XCD's
+    // L2 and waits for it; the block barrier then releases the other waves.
+    // (An acquire in every wave costs 4-8x more per barrier: each is a
+    // cache-wide invalidate.)
+    if (threadIdx.x < 64) {
+        acquire_agent();
+        wait_vm();
+    }
     __syncthreads();
-    acquire_agent();
     return ok_s != 0;
 }
 
 // Persistent batched kernel: batches in order, each as two-phase sub-batches
 // separated by grid barriers.
 template <class L, int G>
-__global__ __launch_bounds__(256) void k_batches(TableDev t, OpsDev o, OutDev out,
+__global__ __launch_bounds__(PERSIST_THREADS) void k_batches(TableDev t, OpsDev o, OutDev out,
                                                  const uint64_t* offsets, uint64_t nbatches,
                                                  uint64_t submax, BarState* bar) {
     const uint64_t ng = (uint64_t)gridDim.x * blockDim.x / G;
@@ -237,7 +249,7 @@ __device__ void serve_schedule(const TableDev& t, const ServeDev& sv, ServeState
 }
 
 template <class L, int G>
-__global__ __launch_bounds__(256) void k_serve(TableDev t, OpsDev o, OutDev out, ServeDev sv,
+__global__ __launch_bounds__(PERSIST_THREADS) void k_serve(TableDev t, OpsDev o, OutDev out, ServeDev sv,
                                                ServeState* ss, BarState* bar, uint64_t submax) {
     const uint64_t ng = (uint64_t)gridDim.x * blockDim.x / G;
     const uint64_t gid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
@@ -408,6 +420,8 @@ static int dispatch(const lslab_table* t, F&& f) {
 template <typename F>
 static int dispatch_g(int g, F&& f) {
     switch (g) {
+        case 1: return f(std::integral_constant<int, 1>{});
+        case 2: return f(std::integral_constant<int, 2>{});
         case 4: return f(std::integral_constant<int, 4>{});
         case 8: return f(std::integral_constant<int, 8>{});
         case 16: return f(std::integral_constant<int, 16>{});
@@ -424,7 +438,7 @@ static void normalise(lslab_tuning& tu) {
 
 static bool tuning_ok(const lslab_tuning& tu) {
     const int g = tu.lanes_per_op;
-    if (g != 4 && g != 8 && g != 16 && g != 32) return false;
+    if (g != 1 && g != 2 && g != 4 && g != 8 && g != 16 && g != 32) return false;
     if (tu.engine != LSLAB_ENGINE_PLAIN && tu.engine != LSLAB_ENGINE_TWO_PHASE) return false;
     if (tu.block_threads < 64 || tu.block_threads > 256 || (tu.block_threads % 64)) return false;
     if (tu.max_blocks < 0) return false;
@@ -735,19 +749,18 @@ extern "C" int lslab_set_mixed(lslab_table* t, const uint8_t* ops, const void* k
 }
 
 // Grid of a persistent kernel whose blocks wait on each other at grid
-// barriers: every block must be resident at once.  The occupancy query counts
-// registers per block but not the SGPR file (a kernel at 64 VGPRs and 106
-// SGPRs fits 7 waves per SIMD, not the 8 its VGPRs allow), so the grid is
-// held to at most 4 blocks of <= 256 threads per CU -- 16 waves per CU, which
-// already saturates random slab reads -- and never above the query.
-static int persistent_grid(const lslab_table* t, const void* kernel, int bt, uint64_t* grid) {
+// barriers: every block must be resident at once.  One block of
+// PERSIST_THREADS (4 waves per SIMD) per CU: the kernels are built for that
+// (launch bounds 1024 cap them at 128 VGPRs; their ~106 SGPRs allow 7 waves
+// per SIMD), and the occupancy query must confirm at least one block per CU.
+// max_blocks (tuning) can shrink the grid for small batches.
+static int persistent_grid(const lslab_table* t, const void* kernel, uint64_t* grid) {
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, bt, 0) != hipSuccess ||
-        per_cu <= 0)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, PERSIST_THREADS, 0) !=
+            hipSuccess ||
+        per_cu < 1)
         return LSLAB_ERR_HIP;
-    const int cap = std::max(1, 1024 / bt);  // <= 16 waves per CU
-    per_cu = std::min(per_cu - (per_cu > 1 ? 1 : 0), cap);
-    uint64_t g = (uint64_t)per_cu * (uint64_t)t->num_cus;
+    uint64_t g = (uint64_t)t->num_cus;
     if (t->tun.max_blocks > 0) g = std::min<uint64_t>(g, (uint64_t)t->tun.max_blocks);
     *grid = g;
     return LSLAB_OK;
@@ -765,17 +778,16 @@ extern "C" int lslab_exec_batches(lslab_table* t, size_t nbatches, const uint64_
     OutDev d = t->is_map ? make_out(out, nullptr, nullptr)
                          : make_out(nullptr, nullptr, static_cast<uint8_t*>(out));
     BarState* bar = &t->state->bar;
-    const int bt = t->tun.block_threads;
     rc = dispatch(t, [&](auto l) {
         using L = decltype(l);
         return dispatch_g(t->tun.lanes_per_op, [&](auto gc) {
             constexpr int G = decltype(gc)::value;
             uint64_t grid = 0;
-            const int rc2 = persistent_grid(t, reinterpret_cast<const void*>(k_batches<L, G>), bt, &grid);
+            const int rc2 = persistent_grid(t, reinterpret_cast<const void*>(k_batches<L, G>), &grid);
             if (rc2) return rc2;
             if (t->tun.max_blocks > 0) grid = std::min<uint64_t>(grid, (uint64_t)t->tun.max_blocks);
             hipLaunchKernelGGL(k_reset_bar, dim3(1), dim3(64), 0, stream, bar);
-            hipLaunchKernelGGL((k_batches<L, G>), dim3((unsigned)grid), dim3(bt), 0, stream, t->dev,
+            hipLaunchKernelGGL((k_batches<L, G>), dim3((unsigned)grid), dim3(PERSIST_THREADS), 0, stream, t->dev,
                                o, d, offsets, (uint64_t)nbatches, t->submax, bar);
             return hipGetLastError() == hipSuccess ? LSLAB_OK : LSLAB_ERR_HIP;
         });
@@ -806,17 +818,16 @@ extern "C" int lslab_serve(lslab_table* t, const lslab_serve_params* p, const ui
                          : make_out(nullptr, nullptr, static_cast<uint8_t*>(out));
     BarState* bar = &t->state->bar;
     ServeState* ss = &t->state->serve;
-    const int bt = t->tun.block_threads;
     rc = dispatch(t, [&](auto l) {
         using L = decltype(l);
         return dispatch_g(t->tun.lanes_per_op, [&](auto gc) {
             constexpr int G = decltype(gc)::value;
             uint64_t grid = 0;
-            const int rc2 = persistent_grid(t, reinterpret_cast<const void*>(k_serve<L, G>), bt, &grid);
+            const int rc2 = persistent_grid(t, reinterpret_cast<const void*>(k_serve<L, G>), &grid);
             if (rc2) return rc2;
             if (t->tun.max_blocks > 0) grid = std::min<uint64_t>(grid, (uint64_t)t->tun.max_blocks);
             hipLaunchKernelGGL(k_reset_serve, dim3(1), dim3(64), 0, stream, ss, bar);
-            hipLaunchKernelGGL((k_serve<L, G>), dim3((unsigned)grid), dim3(bt), 0, stream, t->dev, o,
+            hipLaunchKernelGGL((k_serve<L, G>), dim3((unsigned)grid), dim3(PERSIST_THREADS), 0, stream, t->dev, o,
                                d, sv, ss, bar, t->submax);
             return hipGetLastError() == hipSuccess ? LSLAB_OK : LSLAB_ERR_HIP;
         });

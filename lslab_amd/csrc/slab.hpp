@@ -92,13 +92,12 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
 
 template <int G>
 struct Grp {
-    static_assert(G == 4 || G == 8 || G == 16 || G == 32, "lanes per op");
+    static_assert(G == 1 || G == 2 || G == 4 || G == 8 || G == 16 || G == 32, "lanes per op");
     static constexpr int S = 32 / G;  // key slots per lane
     __device__ static int lane() { return (int)(__lane_id() & (G - 1)); }
     __device__ static int base() { return (int)(__lane_id() & ~(G - 1)); }
     __device__ static uint64_t bits(uint64_t ballot) {
-        if constexpr (G == 64) return ballot;
-        else return (ballot >> base()) & ((1ull << G) - 1ull);
+        return (ballot >> base()) & ((1ull << G) - 1ull);
     }
 };
 

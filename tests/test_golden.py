@@ -67,7 +67,7 @@ def _final(tbl, is_map, kb, vb):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("engine", [0, 1])
-@pytest.mark.parametrize("lanes", [4, 8, 16, 32])
+@pytest.mark.parametrize("lanes", [1, 2, 4, 8, 16, 32])
 @pytest.mark.parametrize("path", FIXTURES, ids=[os.path.basename(p) for p in FIXTURES])
 def test_gpu_reproduces_fixture(lslab, path, engine, lanes):
     import torch

@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 
 TYPES = [(True, 4, 4), (True, 4, 8), (True, 8, 4), (True, 8, 8), (False, 4, 0), (False, 8, 0)]
 ENGINES = [0, 1]
-LANES = [4, 8, 16, 32]
+LANES = [1, 2, 4, 8, 16, 32]
 
 
 def say(*a):
