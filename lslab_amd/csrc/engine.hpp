@@ -206,15 +206,11 @@ __device__ __forceinline__ uint32_t gt_find_or_insert(const TableDev& t, uint64_
 // index order, so a predecessor may have any index of the sub-batch.
 template <class L, int G>
 __device__ __forceinline__ void phase_r_op(const TableDev& t, const OpsDev& o, const OutDev& out,
-                                           uint64_t lo, uint32_t li, uint64_t m) {
-    using K = typename L::K;
+                                           uint64_t lo, uint32_t li, uint64_t m, uint8_t op,
+                                           typename L::K key, typename L::V val) {
     using V = typename L::V;
     const int gl = Grp<G>::lane();
     const uint64_t i = lo + li;
-    uint8_t op;
-    K key;
-    V val;
-    load_op<L>(o, i, op, key, val);
     uint64_t s;
     int slot;
     uint32_t valid;
@@ -286,6 +282,36 @@ __device__ __forceinline__ void phase_r_op(const TableDev& t, const OpsDev& o, c
         write_result<L>(out, i, rf, rv);
         t.op_gidx[li] = g;
         t.op_loc[li] = found ? ((s << 5) | (uint64_t)slot) : NOLOC;
+    }
+}
+
+// Phase R over ops [lo, lo + m) of a sub-batch, group gid of ng: the next
+// op's record (opcode, key, value) is loaded while the current op probes, so
+// the op-record load is off each op's chain of dependent memory accesses.
+template <class L, int G>
+__device__ __forceinline__ void phase_r_range(const TableDev& t, const OpsDev& o,
+                                              const OutDev& out, uint64_t lo, uint64_t m,
+                                              uint64_t gid, uint64_t ng) {
+    using K = typename L::K;
+    using V = typename L::V;
+    uint64_t li = gid;
+    if (li >= m) return;
+    uint8_t op;
+    K key;
+    V val;
+    load_op<L>(o, lo + li, op, key, val);
+    for (;;) {
+        const uint64_t nx = li + ng;
+        uint8_t op2 = 0;
+        K key2 = K(0);
+        V val2 = V(0);
+        if (nx < m) load_op<L>(o, lo + nx, op2, key2, val2);
+        phase_r_op<L, G>(t, o, out, lo, (uint32_t)li, m, op, key, val);
+        if (nx >= m) break;
+        li = nx;
+        op = op2;
+        key = key2;
+        val = val2;
     }
 }
 

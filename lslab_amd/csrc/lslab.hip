@@ -63,7 +63,7 @@ __global__ __launch_bounds__(256) void k_phase_r(TableDev t, OpsDev o, OutDev ou
                                                  uint64_t m) {
     const uint64_t ng = (uint64_t)gridDim.x * blockDim.x / G;
     const uint64_t gid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
-    for (uint64_t li = gid; li < m; li += ng) phase_r_op<L, G>(t, o, out, lo, (uint32_t)li, m);
+    phase_r_range<L, G>(t, o, out, lo, m, gid, ng);
 }
 
 template <class L, int G>
@@ -141,7 +141,7 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_batches(TableDev t, OpsDev 
         const uint64_t lo = offsets[b], hi = offsets[b + 1];
         for (uint64_t slo = lo; slo < hi; slo += submax) {
             const uint64_t m = min(submax, hi - slo);
-            for (uint64_t li = gid; li < m; li += ng) phase_r_op<L, G>(t, o, out, slo, (uint32_t)li, m);
+            phase_r_range<L, G>(t, o, out, slo, m, gid, ng);
             if (!grid_sync(bar, gridDim.x)) {
                 if (threadIdx.x == 0)
                     __hip_atomic_fetch_or(t.status, ST_TIMEOUT, __ATOMIC_RELAXED,
@@ -264,7 +264,7 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_serve(TableDev t, OpsDev o,
             const uint64_t lo = sv.off[r_lo], hi = sv.off[r_hi];
             for (uint64_t slo = lo; slo < hi; slo += submax) {
                 const uint64_t m = min(submax, hi - slo);
-                for (uint64_t li = gid; li < m; li += ng) phase_r_op<L, G>(t, o, out, slo, (uint32_t)li, m);
+                phase_r_range<L, G>(t, o, out, slo, m, gid, ng);
                 if (!grid_sync(bar, gridDim.x)) goto fail;
                 for (uint64_t li = gid; li < m; li += ng) phase_w_op<L, G>(t, o, slo, (uint32_t)li);
                 if (!grid_sync(bar, gridDim.x)) goto fail;
