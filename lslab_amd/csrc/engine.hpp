@@ -200,21 +200,19 @@ __device__ __forceinline__ uint32_t gt_find_or_insert(const TableDev& t, uint64_
     return 0xffffffffu;  // unreachable while a sub-batch holds <= capacity/2 ops
 }
 
-// Phase R of op (lo + li) of a sub-batch of m ops: lock-free lookup of the
-// pre-batch state, result of gets, and the per-key exchange for mutations.
+// Phase R of op (lo + li) of a sub-batch of m ops, given where its lock-free
+// walk of the pre-batch table ended (slab s, slot or -1): result of gets,
+// and the per-key exchange for mutations.
 // The exchange order is the order in which waves reach the atomic, not op
 // index order, so a predecessor may have any index of the sub-batch.
 template <class L, int G>
 __device__ __forceinline__ void phase_r_op(const TableDev& t, const OpsDev& o, const OutDev& out,
                                            uint64_t lo, uint32_t li, uint64_t m, uint8_t op,
-                                           typename L::K key, typename L::V val) {
+                                           typename L::K key, typename L::V val, uint64_t s,
+                                           int slot) {
     using V = typename L::V;
     const int gl = Grp<G>::lane();
     const uint64_t i = lo + li;
-    uint64_t s;
-    int slot;
-    uint32_t valid;
-    walk<L, G, false>(t, (uint64_t)key & t.mask, key, s, slot, valid);
     const bool found = slot >= 0;
     V cur = V(0);
     if constexpr (L::IS_MAP) {
@@ -285,33 +283,59 @@ __device__ __forceinline__ void phase_r_op(const TableDev& t, const OpsDev& o, c
     }
 }
 
-// Phase R over ops [lo, lo + m) of a sub-batch, group gid of ng: the next
-// op's record (opcode, key, value) is loaded while the current op probes, so
-// the op-record load is off each op's chain of dependent memory accesses.
+// Phase R over ops [lo, lo + m) of a sub-batch, group gid of ng, as a
+// three-stage software pipeline per group: while op i finishes (its value
+// load, result, exchange), the head-slab keys of op i+1 and the record
+// (opcode, key, value) of op i+2 are already in flight, so the three
+// dependent round trips of an op overlap with those of its neighbours.
+// Chains longer than the head slab continue synchronously (walk_on).
 template <class L, int G>
 __device__ __forceinline__ void phase_r_range(const TableDev& t, const OpsDev& o,
                                               const OutDev& out, uint64_t lo, uint64_t m,
                                               uint64_t gid, uint64_t ng) {
     using K = typename L::K;
     using V = typename L::V;
+    constexpr int S = Grp<G>::S;
     uint64_t li = gid;
     if (li >= m) return;
     uint8_t op;
     K key;
     V val;
     load_op<L>(o, lo + li, op, key, val);
+    K blk[S];
+    probe_issue<L, G, false>(t, (uint64_t)key & t.mask, blk);
+    uint64_t n1 = li + ng;
+    uint8_t op1 = 0;
+    K key1 = K(0);
+    V val1 = V(0);
+    if (n1 < m) load_op<L>(o, lo + n1, op1, key1, val1);
     for (;;) {
-        const uint64_t nx = li + ng;
+        const uint64_t n2 = n1 + ng;
         uint8_t op2 = 0;
         K key2 = K(0);
         V val2 = V(0);
-        if (nx < m) load_op<L>(o, lo + nx, op2, key2, val2);
-        phase_r_op<L, G>(t, o, out, lo, (uint32_t)li, m, op, key, val);
-        if (nx >= m) break;
-        li = nx;
-        op = op2;
-        key = key2;
-        val = val2;
+        if (n2 < m) load_op<L>(o, lo + n2, op2, key2, val2);
+        const uint64_t b = (uint64_t)key & t.mask;
+        uint64_t s = b;
+        int slot;
+        uint32_t valid;
+        walk_on<L, G, false>(t, probe_compute<L, G, false>(t, b, key, blk), s, slot, valid, key);
+        K blk1[S];
+#pragma unroll
+        for (int e = 0; e < S; ++e) blk1[e] = K(0);
+        if (n1 < m) probe_issue<L, G, false>(t, (uint64_t)key1 & t.mask, blk1);
+        phase_r_op<L, G>(t, o, out, lo, (uint32_t)li, m, op, key, val, s, slot);
+        if (n1 >= m) break;
+        li = n1;
+        op = op1;
+        key = key1;
+        val = val1;
+#pragma unroll
+        for (int e = 0; e < S; ++e) blk[e] = blk1[e];
+        n1 = n2;
+        op1 = op2;
+        key1 = key2;
+        val1 = val2;
     }
 }
 

@@ -201,15 +201,24 @@ struct Probe {
 // lines read by the whole group at once).  Plain loads when the caller
 // guarantees no concurrent writer (read-only launches, phase R); coherent
 // loads (COH) under a bucket lock.
+// Issue half of a probe: lane l of the group loads key slots [l*S, l*S+S).
 template <class L, int G, bool COH>
-__device__ __forceinline__ Probe probe_slab(const TableDev& t, uint64_t s, typename L::K key) {
+__device__ __forceinline__ void probe_issue(const TableDev& t, uint64_t s,
+                                            typename L::K (&mine)[Grp<G>::S]) {
+    using K = typename L::K;
+    constexpr int S = Grp<G>::S;
+    const K* kb = reinterpret_cast<const K*>(slab_ptr<L>(t, s));
+    if constexpr (COH) load_slots_coherent<K, S>(kb + Grp<G>::lane() * S, mine);
+    else load_slots<K, S>(kb + Grp<G>::lane() * S, mine);
+}
+
+// Compute half of a probe over the loaded slots (group-uniform result).
+template <class L, int G, bool COH>
+__device__ __forceinline__ Probe probe_compute(const TableDev& t, uint64_t s, typename L::K key,
+                                               const typename L::K (&mine)[Grp<G>::S]) {
     using K = typename L::K;
     constexpr int S = Grp<G>::S;
     const int gl = Grp<G>::lane();
-    const K* kb = reinterpret_cast<const K*>(slab_ptr<L>(t, s));
-    K mine[S];
-    if constexpr (COH) load_slots_coherent<K, S>(kb + gl * S, mine);
-    else load_slots<K, S>(kb + gl * S, mine);
     Probe p;
     if constexpr (sizeof(K) == 8) {
         uint64_t meta = grp_bcast((uint64_t)mine[S - 1], G - 1, G);
@@ -240,18 +249,21 @@ __device__ __forceinline__ Probe probe_slab(const TableDev& t, uint64_t s, typen
     return p;
 }
 
-// Walks the chain of bucket b: returns the slab/slot holding key, or slot -1
-// with `s` = last slab of the chain and `valid` its bitmap.  COH selects the
-// agent-coherent loads (inside locked sections, where other waves write).
 template <class L, int G, bool COH>
-__device__ __forceinline__ void walk(const TableDev& t, uint64_t b, typename L::K key,
-                                     uint64_t& s, int& slot, uint32_t& valid) {
-    s = b;
+__device__ __forceinline__ Probe probe_slab(const TableDev& t, uint64_t s, typename L::K key) {
+    typename L::K mine[Grp<G>::S];
+    probe_issue<L, G, COH>(t, s, mine);
+    return probe_compute<L, G, COH>(t, s, key, mine);
+}
+
+// Continues a chain walk from probe p of slab s (see walk).
+template <class L, int G, bool COH>
+__device__ __forceinline__ void walk_on(const TableDev& t, Probe p, uint64_t& s, int& slot,
+                                        uint32_t& valid, typename L::K key) {
     // A chain has at most pool + 1 slabs and every link points into the slab
     // array; both are checked so that a damaged chain ends the walk with a
     // status bit instead of a runaway loop or a wild access.
     for (uint64_t hops = 0;; ++hops) {
-        Probe p = probe_slab<L, G, COH>(t, s, key);
         if (p.slot >= 0) {
             slot = p.slot;
             valid = p.valid;
@@ -271,7 +283,18 @@ __device__ __forceinline__ void walk(const TableDev& t, uint64_t b, typename L::
             return;
         }
         s = p.next;
+        p = probe_slab<L, G, COH>(t, s, key);
     }
+}
+
+// Walks the chain of bucket b: returns the slab/slot holding key, or slot -1
+// with `s` = last slab of the chain and `valid` its bitmap.  COH selects the
+// agent-coherent loads (inside locked sections, where other waves write).
+template <class L, int G, bool COH>
+__device__ __forceinline__ void walk(const TableDev& t, uint64_t b, typename L::K key,
+                                     uint64_t& s, int& slot, uint32_t& valid) {
+    s = b;
+    walk_on<L, G, COH>(t, probe_slab<L, G, COH>(t, b, key), s, slot, valid, key);
 }
 
 template <typename T>
