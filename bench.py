@@ -46,8 +46,23 @@ B_OP = P_GET * B_GET + P_PUT * B_PUT  # 296.4
 HBM_PEAK_GBS = 8000.0
 
 
+PROFILE = os.path.join(ROOT, "profiles", "k_batches_map_u64_zipf099_headline.md")
+
+
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def measured_traffic_per_op():
+    """HBM bytes per op of the fused kernel from the committed rocprofv3 PMC
+    summary (FETCH_SIZE x calibration + WRITE_SIZE, per op), or None."""
+    try:
+        for line in open(PROFILE):
+            if line.startswith("traffic_bytes_per_op:"):
+                return float(line.split(":", 1)[1])
+    except (OSError, ValueError):
+        pass
+    return None
 
 
 def make_batches(wl, nb, first_batch):
@@ -391,7 +406,11 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": (round(measured_traffic_per_op() * K * BATCH)
+                        if measured_traffic_per_op() else None),
+            "traffic_source": "rocprofv3 FETCH_SIZE (x2 copy calibration) + WRITE_SIZE per op "
+                              "of k_batches, profiles/k_batches_map_u64_zipf099_headline.md, "
+                              "times the ops of this launch",
             "kernel": "k_batches<Layout<u64,u64,map>, lanes_per_op>",
             "bytes_per_op": B_OP,
         },
