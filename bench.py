@@ -114,12 +114,12 @@ def gen_served(wl, torch, nreq):
     return offs, ops, keys, vals, reserve, rng
 
 
-def _serve_run(lslab_amd, torch, lanes, tick, max_blocks, offs, ops, keys, vals, reserve,
+def _serve_run(lslab_amd, torch, lanes, tick, max_blocks, combine, offs, ops, keys, vals, reserve,
                arrival_h):
     import numpy as np
     mm = lslab_amd.Map(8, 8, 24, 1 << 22,
                        lslab_amd.tuning(lanes_per_op=lanes, engine=lslab_amd.ENGINE_TWO_PHASE,
-                                        max_blocks=max_blocks, max_subbatch=tick))
+                                        max_blocks=max_blocks, max_subbatch=tick, combine=combine))
     arr = torch.from_numpy(arrival_h.astype(np.int64)).cuda()
     torch.cuda.synchronize()
     res = mm.serve(offs, arr, ops, keys, vals, req_reserve=reserve, max_tick_ops=tick)
@@ -130,12 +130,12 @@ def _serve_run(lslab_amd, torch, lanes, tick, max_blocks, offs, ops, keys, vals,
     return res
 
 
-def served_capacity(lslab_amd, wl, torch, lanes, nreq, tick, max_blocks):
+def served_capacity(lslab_amd, wl, torch, lanes, nreq, tick, max_blocks, combine):
     """Mops/s of the serving loop when every request arrives at t=0."""
     import numpy as np
     offs, ops, keys, vals, reserve, _ = gen_served(wl, torch, nreq)
-    res = _serve_run(lslab_amd, torch, lanes, tick, max_blocks, offs, ops, keys, vals, reserve,
-                     np.zeros(nreq, np.int64))
+    res = _serve_run(lslab_amd, torch, lanes, tick, max_blocks, combine, offs, ops, keys, vals,
+                     reserve, np.zeros(nreq, np.int64))
     return round(res["served_ops"] / (res["elapsed_ticks"] / 1e8) / 1e6, 1)
 
 
@@ -146,14 +146,14 @@ def served(lslab_amd, wl, torch, tun, nreq, load_frac=0.7):
     import numpy as np
     offs, ops, keys, vals, reserve, rng = gen_served(wl, torch, nreq)
     n = int(offs[-1])
-    lanes, tick, mb = tun["lanes_per_op"], tun["tick_ops"], tun["max_blocks"]
-    res0 = _serve_run(lslab_amd, torch, lanes, tick, mb, offs, ops, keys, vals, reserve,
+    lanes, tick, mb, cb = tun["lanes_per_op"], tun["tick_ops"], tun["max_blocks"], tun["combine"]
+    res0 = _serve_run(lslab_amd, torch, lanes, tick, mb, cb, offs, ops, keys, vals, reserve,
                       np.zeros(nreq, np.int64))
     cap_s = res0["elapsed_ticks"] / 1e8
     rate = load_frac * nreq / cap_s  # requests per second
     arrival = np.floor(np.cumsum(rng.exponential(1.0 / rate, size=nreq)) * 1e8).astype(np.int64)
     arrival -= arrival[0]
-    res1 = _serve_run(lslab_amd, torch, lanes, tick, mb, offs, ops, keys, vals, reserve, arrival)
+    res1 = _serve_run(lslab_amd, torch, lanes, tick, mb, cb, offs, ops, keys, vals, reserve, arrival)
     lat_us = (res1["complete_ticks"].cpu().numpy() - arrival) / 100.0
     return {"requests": nreq, "ops": n,
             "capacity_mops": round(res0["served_ops"] / cap_s / 1e6, 2),
@@ -162,7 +162,7 @@ def served(lslab_amd, wl, torch, tun, nreq, load_frac=0.7):
             "p50_us": round(float(np.percentile(lat_us, 50)), 2),
             "p99_us": round(float(np.percentile(lat_us, 99)), 2),
             "ticks": int(res1["ticks"]), "rejected": int(res1["rejected"]),
-            "lanes_per_op": lanes, "max_tick_ops": tick, "max_blocks": mb}
+            "lanes_per_op": lanes, "max_tick_ops": tick, "max_blocks": mb, "combine": cb}
 
 
 def time_plain(m, ops, keys, vals, nb, torch):
@@ -238,7 +238,8 @@ def batched(lslab_amd, wl, torch, m, tun, nbatches=4096):
     plain_s = time.perf_counter() - t0
     m.set_tuning(lslab_amd.tuning(lanes_per_op=tun["lanes_per_op"],
                                   engine=lslab_amd.ENGINE_TWO_PHASE,
-                                  max_blocks=tun["max_blocks"], max_subbatch=BATCH))
+                                  max_blocks=tun["max_blocks"], max_subbatch=BATCH,
+                                  combine=tun["combine"]))
     g = torch.cuda.CUDAGraph()
     with torch.cuda.graph(g):
         _lib.check(lib.lslab_exec_batches(m.handle, nbatches, ctypes.c_void_p(offs.data_ptr()),
@@ -259,7 +260,8 @@ def batched(lslab_amd, wl, torch, m, tun, nbatches=4096):
     return {"batches": nbatches, "ops": n, "plain_mops": round(n / plain_s / 1e6, 2),
             "fused_graph_mops": round(n / fused_s / 1e6, 2),
             "fused_vs_plain": round(plain_s / fused_s, 3),
-            "lanes_per_op": tun["lanes_per_op"], "max_blocks": tun["max_blocks"]}
+            "lanes_per_op": tun["lanes_per_op"], "max_blocks": tun["max_blocks"],
+            "combine": tun["combine"]}
 
 
 def cpu_baseline(ops, keys, vals, nb_sample):
@@ -309,6 +311,8 @@ def main():
     ap.add_argument("--steps", type=int, default=32)
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--lanes", type=int, default=0, help="override the tuned lanes per op")
+    ap.add_argument("--combine", type=int, default=0,
+                    help="override the tuned combining scope (1 wavefront, 2 block)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-batches", type=int, default=2)
     ap.add_argument("--no-batched", action="store_true")
@@ -330,9 +334,12 @@ def main():
     torch.cuda.set_device(0)
     K, W = args.steps, args.warmup
     tuned = {k: autotune.load(k) for k in ("headline", "batched", "served")}
-    if args.lanes:
-        for v in tuned.values():
+    for v in tuned.values():
+        v.setdefault("combine", lslab_amd.COMBINE_BLOCK)
+        if args.lanes:
             v["lanes_per_op"] = args.lanes
+        if args.combine:
+            v["combine"] = args.combine
     if args.copy_calibration:
         src = torch.ones(1 << 27, dtype=torch.int64, device="cuda")
         dst = torch.empty_like(src)
@@ -360,7 +367,8 @@ def main():
         raise RuntimeError(f"plain status {st}")
 
     m.set_tuning(lslab_amd.tuning(lanes_per_op=th["lanes_per_op"], engine=lslab_amd.ENGINE_TWO_PHASE,
-                                  max_blocks=th["max_blocks"], max_subbatch=BATCH))
+                                  max_blocks=th["max_blocks"], max_subbatch=BATCH,
+                                  combine=th["combine"]))
     time_fused(m, ops, keys, vals, offs[: W + 1], W, torch)
     offs_k = offs[W:] - W * BATCH
     wall_f, kern_f = time_fused(m, ops[W * BATCH:], keys[W * BATCH:], vals[W * BATCH:], offs_k, K,
@@ -392,6 +400,7 @@ def main():
             "engine": "two-phase, one persistent launch for all timed batches",
             "lanes_per_op": th["lanes_per_op"],
             "max_blocks": th["max_blocks"],
+            "combine": th["combine"],
             "global_batch": BATCH,
             "parallelism": "single GPU",
         },

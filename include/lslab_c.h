@@ -54,6 +54,13 @@ extern "C" {
 #define LSLAB_ENGINE_PLAIN 0
 #define LSLAB_ENGINE_TWO_PHASE 1
 
+/* Two-phase combining scope: mutations of one key are chained within a
+ * wavefront (WAVE), or also across the wavefronts of a block through LDS
+ * (BLOCK), before the one global exchange per chain.  BLOCK adds one block
+ * barrier per round and pays off under heavy key skew. */
+#define LSLAB_COMBINE_WAVE 1
+#define LSLAB_COMBINE_BLOCK 2
+
 typedef struct lslab_table lslab_table;
 
 typedef struct {
@@ -63,6 +70,7 @@ typedef struct {
     int max_blocks;        /* grid cap for the per-batch kernels (0 = auto) */
     uint64_t max_subbatch; /* two-phase ops per sub-batch; sizes the combining
                               table and per-op scratch (0 = 2^20) */
+    int combine;           /* LSLAB_COMBINE_* for the two-phase engine (0 = BLOCK) */
 } lslab_tuning;
 
 typedef struct {

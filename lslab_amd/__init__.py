@@ -5,11 +5,11 @@ in include/lslab_c.h) and the header-only C++ API in include/lslab/.  This
 Python package binds the same C ABI for tests and benchmarks.
 """
 from . import _lib  # noqa: F401
-from ._lib import (ENGINE_PLAIN, ENGINE_TWO_PHASE, OP_GET, OP_PUT, OP_REMOVE,  # noqa: F401
-                   LslabError)
+from ._lib import (COMBINE_BLOCK, COMBINE_WAVE, ENGINE_PLAIN, ENGINE_TWO_PHASE,  # noqa: F401
+                   OP_GET, OP_PUT, OP_REMOVE, LslabError)
 
-__all__ = ["Map", "Set", "tuning", "workload", "ENGINE_PLAIN", "ENGINE_TWO_PHASE", "OP_GET",
-           "OP_PUT", "OP_REMOVE", "LslabError"]
+__all__ = ["Map", "Set", "tuning", "workload", "ENGINE_PLAIN", "ENGINE_TWO_PHASE",
+           "COMBINE_WAVE", "COMBINE_BLOCK", "OP_GET", "OP_PUT", "OP_REMOVE", "LslabError"]
 
 
 def __getattr__(name):

@@ -23,6 +23,9 @@ OP_REMOVE = 2
 ENGINE_PLAIN = 0
 ENGINE_TWO_PHASE = 1
 
+COMBINE_WAVE = 1
+COMBINE_BLOCK = 2
+
 # Every symbol include/lslab_c.h declares.
 EXPORTS = (
     "lslab_map_create", "lslab_set_create", "lslab_destroy", "lslab_set_tuning",
@@ -42,6 +45,7 @@ class Tuning(ctypes.Structure):
         ("block_threads", ctypes.c_int),
         ("max_blocks", ctypes.c_int),
         ("max_subbatch", ctypes.c_uint64),
+        ("combine", ctypes.c_int),
     ]
 
 

@@ -9,6 +9,8 @@ For each timed workload it sweeps, on a short sample of that workload:
   - max_blocks   : persistent grid size (one 1024-thread block per CU at
                    most); fewer blocks make each grid barrier cheaper, which
                    pays off for small batches.
+  - combine      : combining scope of same-key mutations before the global
+                   exchange (1 = within a wavefront, 2 = across the block);
   - tick ops     : (served form) ops admitted per tick.
 and keeps the setting with the highest Mops/s.  Results go to
 lslab_amd/tuning.json under the workload's name; bench.py reads them.
@@ -26,9 +28,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 TUNING_JSON = os.path.join(HERE, "tuning.json")
 
 DEFAULTS = {
-    "headline": {"lanes_per_op": 2, "max_blocks": 0},
-    "batched": {"lanes_per_op": 4, "max_blocks": 0},
-    "served": {"lanes_per_op": 2, "max_blocks": 0, "tick_ops": 1 << 18},
+    "headline": {"lanes_per_op": 1, "max_blocks": 0, "combine": 1},
+    "batched": {"lanes_per_op": 8, "max_blocks": 128, "combine": 2},
+    "served": {"lanes_per_op": 2, "max_blocks": 128, "tick_ops": 1 << 18, "combine": 2},
 }
 
 
@@ -73,15 +75,16 @@ def search_batches(table, offs, ops, keys, vals, nb, n_ops, max_subbatch, lanes,
     from . import _lib, table as tb
     out = torch.empty((n_ops, 2), dtype=torch.int64, device="cuda")
     res = []
-    for g, mb in itertools.product(lanes, grids):
+    for g, mb, cb in itertools.product(lanes, grids, (_lib.COMBINE_WAVE, _lib.COMBINE_BLOCK)):
         table.set_tuning(tb.tuning(lanes_per_op=g, engine=_lib.ENGINE_TWO_PHASE, max_blocks=mb,
-                                   max_subbatch=max_subbatch))
+                                   max_subbatch=max_subbatch, combine=cb))
         dt = _timed(lambda: _exec(table, offs, ops, keys, vals, out, nb))
         st = table.status()
         if st:
             raise RuntimeError(f"status {st} during autotune")
-        res.append({"lanes_per_op": g, "max_blocks": mb, "mops": round(n_ops / dt / 1e6, 1)})
-        log(f"[autotune] lanes={g} blocks={mb or 'all'}: {res[-1]['mops']} Mops/s")
+        res.append({"lanes_per_op": g, "max_blocks": mb, "combine": cb,
+                    "mops": round(n_ops / dt / 1e6, 1)})
+        log(f"[autotune] lanes={g} blocks={mb or 'all'} combine={cb}: {res[-1]['mops']} Mops/s")
     return max(res, key=lambda r: r["mops"]), res
 
 
@@ -114,10 +117,13 @@ def main():
     m.close()
     # served: loop capacity (every request arrives at t=0) on a short stream
     res = []
-    for g, tick, mb in itertools.product((2, 4), (1 << 16, 1 << 17, 1 << 18), (0, 128)):
-        sv = bench.served_capacity(lslab_amd, wl, torch, g, 4000, tick, mb)
-        res.append({"lanes_per_op": g, "tick_ops": tick, "max_blocks": mb, "mops": sv})
-        log(f"[autotune] served lanes={g} tick={tick} blocks={mb or 'all'}: {sv} Mops/s")
+    for g, tick, mb, cb in itertools.product((1, 2, 4), (1 << 16, 1 << 17, 1 << 18), (0, 128),
+                                             (lslab_amd.COMBINE_WAVE, lslab_amd.COMBINE_BLOCK)):
+        sv = bench.served_capacity(lslab_amd, wl, torch, g, 4000, tick, mb, cb)
+        res.append({"lanes_per_op": g, "tick_ops": tick, "max_blocks": mb, "combine": cb,
+                    "mops": sv})
+        log(f"[autotune] served lanes={g} tick={tick} blocks={mb or 'all'} combine={cb}: "
+            f"{sv} Mops/s")
     top = max(res, key=lambda r: r["mops"])
     out["served"] = dict(top, all=res)
     with open(TUNING_JSON, "w") as f:

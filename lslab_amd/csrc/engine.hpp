@@ -200,38 +200,152 @@ __device__ __forceinline__ uint32_t gt_find_or_insert(const TableDev& t, uint64_
     return 0xffffffffu;  // unreachable while a sub-batch holds <= capacity/2 ops
 }
 
+// Block-level combining table for one phase-R round.  Two tables alternate
+// by round parity: the entries a round uses are cleared in that round, and a
+// table is reused two rounds (two block barriers) later.
+template <int BT>
+struct CombTab {
+    static constexpr uint32_t N = 2u * BT;  // power of two; a round has at most BT chains
+    uint64_t key[N + 1];                    // entry N is reserved for the all-ones key
+    uint32_t head[N + 1];                   // last chain so far: its last op index + 1
+    uint32_t first[N + 1];                  // first op of the block's first chain
+};
+
+template <int BT>
+__device__ __forceinline__ CombTab<BT>* comb_tables() {
+    __shared__ CombTab<BT> tabs[2];
+    return tabs;
+}
+
+template <int BT>
+__device__ __forceinline__ void comb_init() {
+    CombTab<BT>* tb = comb_tables<BT>();
+    constexpr uint32_t E = CombTab<BT>::N + 1;
+    for (uint32_t x = threadIdx.x; x < 2 * E; x += blockDim.x) {
+        tb[x / E].key[x % E] = GT_EMPTY;
+        tb[x / E].head[x % E] = 0;
+    }
+    __syncthreads();
+}
+
+// Entry of key in the round's table (inserted if new; linear probing).
+template <int BT>
+__device__ __forceinline__ uint32_t comb_slot(CombTab<BT>& c, uint64_t key) {
+    constexpr uint32_t N = CombTab<BT>::N;
+    if (key == GT_EMPTY) return N;
+    uint32_t h = (uint32_t)mix64(key) & (N - 1u);
+    for (uint32_t p = 0; p < N; ++p) {
+        const uint64_t cur = __hip_atomic_load(&c.key[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (cur == key) return h;
+        if (cur == GT_EMPTY) {
+            uint64_t e = GT_EMPTY;
+            if (__hip_atomic_compare_exchange_strong(&c.key[h], &e, key, __ATOMIC_RELAXED,
+                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
+                return h;
+            if (e == key) return h;
+        }
+        h = (h + 1u) & (N - 1u);
+    }
+    return N;  // unreachable: the table is at most half full
+}
+
+// The one global step of a chain: find or insert the key's combining-table
+// entry and swap the chain's last op (index + 1) into its head.  pred gets
+// the previous head: the last op of the chain exchanged before, or 0.
+__device__ __forceinline__ uint32_t gt_exchange(const TableDev& t, uint64_t key, uint32_t v,
+                                                uint32_t& pred) {
+    const uint32_t g = gt_find_or_insert(t, key);
+    if (g != 0xffffffffu) {
+        pred = __hip_atomic_exchange(&t.gt_head[g], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        pred = 0;
+        __hip_atomic_fetch_or(t.status, ST_CORRUPT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return g;
+}
+
+// Writes the result of mutation (lo + li) with opcode op, whose serial
+// predecessor in the sub-batch is op (lo + pred - 1), or, for pred == 0, the
+// key's pre-batch state (found, cur).
+template <class L>
+__device__ __forceinline__ void mut_result(const TableDev& t, const OpsDev& o, const OutDev& out,
+                                           uint64_t lo, uint64_t m, uint32_t li, uint8_t op,
+                                           uint32_t pred, bool found, typename L::V cur) {
+    using V = typename L::V;
+    bool rf = found;
+    V rv = cur;
+    if (pred > m) {  // entries only ever hold ops of this sub-batch (any order)
+        __hip_atomic_fetch_or(t.status, ST_CORRUPT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        pred = 0;
+    }
+    if (pred != 0) {
+        const uint64_t pi = lo + pred - 1u;
+        const uint8_t pop = o.ops ? o.ops[pi] : o.uop;
+        if (pop == OP_PUT) {
+            rf = true;
+            rv = L::IS_MAP ? ld_stride<V>(o.vals, o.vstride, pi) : V(0);
+        } else {
+            rf = false;
+            rv = V(0);
+        }
+    }
+    if (!L::IS_MAP && op == OP_PUT) rf = true;  // set insert reports true
+    write_result<L>(out, lo + li, rf, rv);
+}
+
 // Phase R of op (lo + li) of a sub-batch of m ops, given where its lock-free
-// walk of the pre-batch table ended (slab s, slot or -1): result of gets,
-// and the per-key exchange for mutations.
-// The exchange order is the order in which waves reach the atomic, not op
-// index order, so a predecessor may have any index of the sub-batch.
-template <class L, int G>
+// walk of the pre-batch table ended (slab s, slot or -1): the result of gets,
+// and for mutations their place in the key's serial order.
+//
+// Mutations of one key form a chain: within a wavefront in lane order, then
+// (block combining) the wavefront chains of the block in the order they
+// reach the round's LDS table, then the block chains in the order of their
+// one global exchange.  Each op's result comes from its predecessor in the
+// chain; the first op of the block's first chain learns its predecessor only
+// from the global exchange, so the block's last chain, which does that
+// exchange, writes it.  The global exchange order is not op index order, so
+// a predecessor may have any index of the sub-batch.
+//
+// Only the last op of each block chain (wave chain without block combining)
+// can be the key's final mutation, so only it records its combining-table
+// entry and pre-batch location for phase W; the other mutations record none.
+//
+// With block combining (BC; ct is the round's table) every thread of the
+// block calls this once per round (idle groups with act = false), as it
+// contains a block barrier.
+template <class L, int G, int BT, bool BC>
 __device__ __forceinline__ void phase_r_op(const TableDev& t, const OpsDev& o, const OutDev& out,
-                                           uint64_t lo, uint32_t li, uint64_t m, uint8_t op,
-                                           typename L::K key, typename L::V val, uint64_t s,
-                                           int slot) {
+                                           uint64_t lo, uint32_t li, uint64_t m, bool act,
+                                           uint8_t op, typename L::K key, typename L::V val,
+                                           uint64_t s, int slot, CombTab<BT>* ct) {
     using V = typename L::V;
     const int gl = Grp<G>::lane();
-    const uint64_t i = lo + li;
-    const bool found = slot >= 0;
+    const bool found = act && slot >= 0;
     V cur = V(0);
     if constexpr (L::IS_MAP) {
         if (found) cur = *val_slot<L>(t, s, slot);
     }
-    if (op == OP_GET) {
-        if (gl == 0) write_result<L>(out, i, found, cur);
-        return;
+    if constexpr (BC) {
+        if (act && op == OP_GET && gl == 0) write_result<L>(out, lo + li, found, cur);
+    } else {  // no block barrier ahead: gets are done
+        if (op == OP_GET) {
+            if (gl == 0) write_result<L>(out, lo + li, found, cur);
+            return;
+        }
     }
-    // Wave-level combining: the mutation leaders of this wave that share a
-    // key are chained in lane order, and only the last of them touches the
-    // key's combining entry (one lookup, one exchange per key per wave instead
-    // of per op -- a Zipf-hot key's exchanges are what serialise).
     const int lane = (int)__lane_id();
-    const bool lead = (gl == 0);
+    const bool lead = act && op != OP_GET && gl == 0;
     const uint64_t kk = (uint64_t)key;
-    int my_last = lane, my_prev = -1;
     uint64_t todo = __ballot(lead);
-    while (todo) {  // wave-uniform: one pass per distinct key among the leaders
+    if constexpr (BC) {
+        if (todo == 0) {  // wave-uniform: no mutation in this wavefront
+            lds_barrier();
+            return;
+        }
+    }
+    // Wave chains: one pass per distinct key among the mutation leaders.
+    int my_first = lane, my_last = lane, my_prev = -1;
+    while (todo) {
         const int l = __builtin_ctzll(todo);
         const uint64_t kl = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(kk >> 32), l) << 32) |
                             (uint32_t)__builtin_amdgcn_readlane((uint32_t)kk, l);
@@ -239,47 +353,50 @@ __device__ __forceinline__ void phase_r_op(const TableDev& t, const OpsDev& o, c
         const uint64_t same = __ballot(mine);
         todo &= ~same;
         if (mine) {
+            if constexpr (BC) my_first = __builtin_ctzll(same);
             my_last = 63 - __builtin_clzll(same);
             const uint64_t below = same & ((1ull << lane) - 1ull);
             my_prev = below ? 63 - __builtin_clzll(below) : -1;
         }
     }
-    uint32_t g = 0, pred = 0;
-    if (lead && lane == my_last) {
-        g = gt_find_or_insert(t, kk);
-        if (g != 0xffffffffu) {
-            pred = __hip_atomic_exchange(&t.gt_head[g], li + 1u, __ATOMIC_RELAXED,
-                                         __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            __hip_atomic_fetch_or(t.status, ST_CORRUPT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const bool rep = lead && lane == my_last;  // represents its wave chain
+    uint32_t g = 0xffffffffu, fpred = 0;
+    bool fdefer = false;  // the chain's first result is written by the block's last chain
+    if constexpr (BC) {
+        const uint32_t first_li = __shfl(li, my_first, 64);
+        uint32_t j = 0;
+        if (rep) {
+            j = comb_slot<BT>(*ct, kk);
+            fpred = __hip_atomic_exchange(&ct->head[j], li + 1u, __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (fpred == 0) ct->first[j] = first_li;
         }
-    }
-    g = __shfl(g, my_last, 64);
-    pred = __shfl(pred, my_last, 64);
-    const uint32_t prev_li = __shfl(li, my_prev >= 0 ? my_prev : lane, 64);
-    if (my_prev >= 0) pred = prev_li + 1u;
-    if (lead) {
-        bool rf = found;
-        V rv = cur;
-        if (pred > m) {  // entries only ever hold ops of this sub-batch (any order)
-            __hip_atomic_fetch_or(t.status, ST_CORRUPT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            pred = 0;
-        }
-        if (pred != 0) {
-            const uint64_t pi = lo + pred - 1u;
-            const uint8_t pop = o.ops ? o.ops[pi] : o.uop;
-            if (pop == OP_PUT) {
-                rf = true;
-                rv = L::IS_MAP ? ld_stride<V>(o.vals, o.vstride, pi) : V(0);
-            } else {
-                rf = false;
-                rv = V(0);
+        lds_barrier();
+        if (rep) {
+            fdefer = fpred == 0;
+            if (ct->head[j] == li + 1u) {  // the block's last chain of this key
+                const uint32_t fl = ct->first[j];
+                ct->key[j] = GT_EMPTY;
+                ct->head[j] = 0;
+                uint32_t P;
+                g = gt_exchange(t, kk, li + 1u, P);
+                uint8_t fop = OP_PUT;
+                if constexpr (!L::IS_MAP) fop = o.ops ? o.ops[lo + fl] : o.uop;
+                mut_result<L>(t, o, out, lo, m, fl, fop, P, found, cur);
             }
         }
-        if (!L::IS_MAP && op == OP_PUT) rf = true;  // set insert reports true
-        write_result<L>(out, i, rf, rv);
+    } else if (rep) {
+        g = gt_exchange(t, kk, li + 1u, fpred);
+    }
+    fpred = __shfl(fpred, my_last, 64);
+    if constexpr (BC) fdefer = __shfl((int)fdefer, my_last, 64) != 0;
+    const uint32_t prev_li = __shfl(li, my_prev >= 0 ? my_prev : lane, 64);
+    if (lead) {
+        if (my_prev >= 0 || !fdefer)
+            mut_result<L>(t, o, out, lo, m, li, op, my_prev >= 0 ? prev_li + 1u : fpred, found,
+                          cur);
         t.op_gidx[li] = g;
-        t.op_loc[li] = found ? ((s << 5) | (uint64_t)slot) : NOLOC;
+        if (rep) t.op_loc[li] = found ? ((s << 5) | (uint64_t)slot) : NOLOC;
     }
 }
 
@@ -288,44 +405,64 @@ __device__ __forceinline__ void phase_r_op(const TableDev& t, const OpsDev& o, c
 // load, result, exchange), the head-slab keys of op i+1 and the record
 // (opcode, key, value) of op i+2 are already in flight, so the three
 // dependent round trips of an op overlap with those of its neighbours.
-// Chains longer than the head slab continue synchronously (walk_on).
-template <class L, int G>
+// Chains longer than the head slab continue synchronously (walk_on).  All
+// groups of a block run the same number of rounds (idle ones inactive), as
+// the block-level combining in phase_r_op requires.
+template <class L, int G, int BT, bool BC>
 __device__ __forceinline__ void phase_r_range(const TableDev& t, const OpsDev& o,
                                               const OutDev& out, uint64_t lo, uint64_t m,
                                               uint64_t gid, uint64_t ng) {
     using K = typename L::K;
     using V = typename L::V;
     constexpr int S = Grp<G>::S;
+    uint64_t rounds = 0;  // BC: rounds of the block's first group, the most of any
+    CombTab<BT>* tabs = nullptr;
+    if constexpr (BC) {
+        const uint64_t gid0 = (uint64_t)blockIdx.x * blockDim.x / G;
+        if (gid0 >= m) return;  // block-uniform
+        rounds = (m - gid0 + ng - 1) / ng;
+        tabs = comb_tables<BT>();
+    } else {
+        if (gid >= m) return;
+    }
     uint64_t li = gid;
-    if (li >= m) return;
-    uint8_t op;
-    K key;
-    V val;
-    load_op<L>(o, lo + li, op, key, val);
+    uint8_t op = OP_GET;
+    K key = K(0);
+    V val = V(0);
+    if (!BC || li < m) load_op<L>(o, lo + li, op, key, val);
     K blk[S];
-    probe_issue<L, G, false>(t, (uint64_t)key & t.mask, blk);
+#pragma unroll
+    for (int e = 0; e < S; ++e) blk[e] = K(0);
+    if (!BC || li < m) probe_issue<L, G, false>(t, (uint64_t)key & t.mask, blk);
     uint64_t n1 = li + ng;
-    uint8_t op1 = 0;
+    uint8_t op1 = OP_GET;
     K key1 = K(0);
     V val1 = V(0);
     if (n1 < m) load_op<L>(o, lo + n1, op1, key1, val1);
-    for (;;) {
+    for (uint64_t r = 0;; ++r) {
+        if constexpr (BC) {
+            if (r >= rounds) break;
+        }
         const uint64_t n2 = n1 + ng;
-        uint8_t op2 = 0;
+        uint8_t op2 = OP_GET;
         K key2 = K(0);
         V val2 = V(0);
         if (n2 < m) load_op<L>(o, lo + n2, op2, key2, val2);
         const uint64_t b = (uint64_t)key & t.mask;
         uint64_t s = b;
-        int slot;
+        int slot = -1;
         uint32_t valid;
-        walk_on<L, G, false>(t, probe_compute<L, G, false>(t, b, key, blk), s, slot, valid, key);
+        if (!BC || li < m)
+            walk_on<L, G, false>(t, probe_compute<L, G, false>(t, b, key, blk), s, slot, valid, key);
         K blk1[S];
 #pragma unroll
         for (int e = 0; e < S; ++e) blk1[e] = K(0);
         if (n1 < m) probe_issue<L, G, false>(t, (uint64_t)key1 & t.mask, blk1);
-        phase_r_op<L, G>(t, o, out, lo, (uint32_t)li, m, op, key, val, s, slot);
-        if (n1 >= m) break;
+        phase_r_op<L, G, BT, BC>(t, o, out, lo, (uint32_t)li, m, !BC || li < m, op, key, val, s,
+                                 slot, BC ? tabs + (r & 1) : nullptr);
+        if constexpr (!BC) {
+            if (n1 >= m) break;
+        }
         li = n1;
         op = op1;
         key = key1;

@@ -58,12 +58,13 @@ __global__ __launch_bounds__(256) void k_plain(TableDev t, OpsDev o, OutDev out,
     }
 }
 
-template <class L, int G>
+template <class L, int G, bool BC>
 __global__ __launch_bounds__(256) void k_phase_r(TableDev t, OpsDev o, OutDev out, uint64_t lo,
                                                  uint64_t m) {
     const uint64_t ng = (uint64_t)gridDim.x * blockDim.x / G;
     const uint64_t gid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
-    phase_r_range<L, G>(t, o, out, lo, m, gid, ng);
+    if constexpr (BC) comb_init<256>();
+    phase_r_range<L, G, 256, BC>(t, o, out, lo, m, gid, ng);
 }
 
 template <class L, int G>
@@ -131,17 +132,18 @@ __device__ __forceinline__ bool grid_sync(BarState* bar, uint32_t nblocks) {
 
 // Persistent batched kernel: batches in order, each as two-phase sub-batches
 // separated by grid barriers.
-template <class L, int G>
+template <class L, int G, bool BC>
 __global__ __launch_bounds__(PERSIST_THREADS) void k_batches(TableDev t, OpsDev o, OutDev out,
                                                  const uint64_t* offsets, uint64_t nbatches,
                                                  uint64_t submax, BarState* bar) {
     const uint64_t ng = (uint64_t)gridDim.x * blockDim.x / G;
     const uint64_t gid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+    if constexpr (BC) comb_init<PERSIST_THREADS>();
     for (uint64_t b = 0; b < nbatches; ++b) {
         const uint64_t lo = offsets[b], hi = offsets[b + 1];
         for (uint64_t slo = lo; slo < hi; slo += submax) {
             const uint64_t m = min(submax, hi - slo);
-            phase_r_range<L, G>(t, o, out, slo, m, gid, ng);
+            phase_r_range<L, G, PERSIST_THREADS, BC>(t, o, out, slo, m, gid, ng);
             if (!grid_sync(bar, gridDim.x)) {
                 if (threadIdx.x == 0)
                     __hip_atomic_fetch_or(t.status, ST_TIMEOUT, __ATOMIC_RELAXED,
@@ -248,12 +250,13 @@ __device__ void serve_schedule(const TableDev& t, const ServeDev& sv, ServeState
     if (n == 0) st_coh(&ss->done, 1u);
 }
 
-template <class L, int G>
+template <class L, int G, bool BC>
 __global__ __launch_bounds__(PERSIST_THREADS) void k_serve(TableDev t, OpsDev o, OutDev out, ServeDev sv,
                                                ServeState* ss, BarState* bar, uint64_t submax) {
     const uint64_t ng = (uint64_t)gridDim.x * blockDim.x / G;
     const uint64_t gid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
     __shared__ uint64_t stamp_s;
+    if constexpr (BC) comb_init<PERSIST_THREADS>();
     if (blockIdx.x == 0 && threadIdx.x == 0) ss->t0 = now_ticks();
     for (uint64_t tick = 0;; ++tick) {
         if (blockIdx.x == 0 && threadIdx.x == 0) serve_schedule(t, sv, ss);
@@ -264,7 +267,7 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_serve(TableDev t, OpsDev o,
             const uint64_t lo = sv.off[r_lo], hi = sv.off[r_hi];
             for (uint64_t slo = lo; slo < hi; slo += submax) {
                 const uint64_t m = min(submax, hi - slo);
-                phase_r_range<L, G>(t, o, out, slo, m, gid, ng);
+                phase_r_range<L, G, PERSIST_THREADS, BC>(t, o, out, slo, m, gid, ng);
                 if (!grid_sync(bar, gridDim.x)) goto fail;
                 for (uint64_t li = gid; li < m; li += ng) phase_w_op<L, G>(t, o, slo, (uint32_t)li);
                 if (!grid_sync(bar, gridDim.x)) goto fail;
@@ -430,10 +433,18 @@ static int dispatch_g(int g, F&& f) {
     return LSLAB_ERR_INVALID;
 }
 
+// Combining scope as a compile-time flag: true = block, false = wavefront.
+template <class F>
+static int dispatch_c(int combine, F&& f) {
+    if (combine == LSLAB_COMBINE_BLOCK) return f(std::true_type{});
+    return f(std::false_type{});
+}
+
 static void normalise(lslab_tuning& tu) {
     if (tu.lanes_per_op == 0) tu.lanes_per_op = 4;
     if (tu.block_threads == 0) tu.block_threads = 256;
     if (tu.max_subbatch == 0) tu.max_subbatch = 1ull << 20;
+    if (tu.combine == 0) tu.combine = LSLAB_COMBINE_BLOCK;
 }
 
 static bool tuning_ok(const lslab_tuning& tu) {
@@ -443,6 +454,7 @@ static bool tuning_ok(const lslab_tuning& tu) {
     if (tu.block_threads < 64 || tu.block_threads > 256 || (tu.block_threads % 64)) return false;
     if (tu.max_blocks < 0) return false;
     if (tu.max_subbatch < 64 || tu.max_subbatch > (1ull << 30)) return false;
+    if (tu.combine != LSLAB_COMBINE_WAVE && tu.combine != LSLAB_COMBINE_BLOCK) return false;
     return true;
 }
 
@@ -649,8 +661,12 @@ static int run_ops(lslab_table* t, const OpsDev& o, const OutDev& out, uint64_t 
                 for (uint64_t lo = 0; lo < n; lo += t->submax) {
                     const uint64_t m = std::min<uint64_t>(t->submax, n - lo);
                     const uint64_t grid = grid_for(t, m);
-                    hipLaunchKernelGGL((k_phase_r<L, G>), dim3((unsigned)grid), dim3(bt), 0, s,
-                                       t->dev, o, out, lo, m);
+                    if (t->tun.combine == LSLAB_COMBINE_BLOCK)
+                        hipLaunchKernelGGL((k_phase_r<L, G, true>), dim3((unsigned)grid), dim3(bt),
+                                           0, s, t->dev, o, out, lo, m);
+                    else
+                        hipLaunchKernelGGL((k_phase_r<L, G, false>), dim3((unsigned)grid),
+                                           dim3(bt), 0, s, t->dev, o, out, lo, m);
                     hipLaunchKernelGGL((k_phase_w<L, G>), dim3((unsigned)grid), dim3(bt), 0, s,
                                        t->dev, o, lo, m);
                 }
@@ -782,14 +798,20 @@ extern "C" int lslab_exec_batches(lslab_table* t, size_t nbatches, const uint64_
         using L = decltype(l);
         return dispatch_g(t->tun.lanes_per_op, [&](auto gc) {
             constexpr int G = decltype(gc)::value;
-            uint64_t grid = 0;
-            const int rc2 = persistent_grid(t, reinterpret_cast<const void*>(k_batches<L, G>), &grid);
-            if (rc2) return rc2;
-            if (t->tun.max_blocks > 0) grid = std::min<uint64_t>(grid, (uint64_t)t->tun.max_blocks);
-            hipLaunchKernelGGL(k_reset_bar, dim3(1), dim3(64), 0, stream, bar);
-            hipLaunchKernelGGL((k_batches<L, G>), dim3((unsigned)grid), dim3(PERSIST_THREADS), 0, stream, t->dev,
-                               o, d, offsets, (uint64_t)nbatches, t->submax, bar);
-            return hipGetLastError() == hipSuccess ? LSLAB_OK : LSLAB_ERR_HIP;
+            return dispatch_c(t->tun.combine, [&](auto cc) {
+                constexpr bool BC = decltype(cc)::value;
+                uint64_t grid = 0;
+                const int rc2 =
+                    persistent_grid(t, reinterpret_cast<const void*>(k_batches<L, G, BC>), &grid);
+                if (rc2) return rc2;
+                if (t->tun.max_blocks > 0)
+                    grid = std::min<uint64_t>(grid, (uint64_t)t->tun.max_blocks);
+                hipLaunchKernelGGL(k_reset_bar, dim3(1), dim3(64), 0, stream, bar);
+                hipLaunchKernelGGL((k_batches<L, G, BC>), dim3((unsigned)grid),
+                                   dim3(PERSIST_THREADS), 0, stream, t->dev, o, d, offsets,
+                                   (uint64_t)nbatches, t->submax, bar);
+                return hipGetLastError() == hipSuccess ? LSLAB_OK : LSLAB_ERR_HIP;
+            });
         });
     });
     if (rc) return rc;
@@ -822,14 +844,20 @@ extern "C" int lslab_serve(lslab_table* t, const lslab_serve_params* p, const ui
         using L = decltype(l);
         return dispatch_g(t->tun.lanes_per_op, [&](auto gc) {
             constexpr int G = decltype(gc)::value;
-            uint64_t grid = 0;
-            const int rc2 = persistent_grid(t, reinterpret_cast<const void*>(k_serve<L, G>), &grid);
-            if (rc2) return rc2;
-            if (t->tun.max_blocks > 0) grid = std::min<uint64_t>(grid, (uint64_t)t->tun.max_blocks);
-            hipLaunchKernelGGL(k_reset_serve, dim3(1), dim3(64), 0, stream, ss, bar);
-            hipLaunchKernelGGL((k_serve<L, G>), dim3((unsigned)grid), dim3(PERSIST_THREADS), 0, stream, t->dev, o,
-                               d, sv, ss, bar, t->submax);
-            return hipGetLastError() == hipSuccess ? LSLAB_OK : LSLAB_ERR_HIP;
+            return dispatch_c(t->tun.combine, [&](auto cc) {
+                constexpr bool BC = decltype(cc)::value;
+                uint64_t grid = 0;
+                const int rc2 =
+                    persistent_grid(t, reinterpret_cast<const void*>(k_serve<L, G, BC>), &grid);
+                if (rc2) return rc2;
+                if (t->tun.max_blocks > 0)
+                    grid = std::min<uint64_t>(grid, (uint64_t)t->tun.max_blocks);
+                hipLaunchKernelGGL(k_reset_serve, dim3(1), dim3(64), 0, stream, ss, bar);
+                hipLaunchKernelGGL((k_serve<L, G, BC>), dim3((unsigned)grid),
+                                   dim3(PERSIST_THREADS), 0, stream, t->dev, o, d, sv, ss, bar,
+                                   t->submax);
+                return hipGetLastError() == hipSuccess ? LSLAB_OK : LSLAB_ERR_HIP;
+            });
         });
     });
     if (rc) return rc;

@@ -167,6 +167,15 @@ __device__ __forceinline__ void st_coh(T* p, T v) {
 // lock release or barrier arrival needs (no cache-wide write-back).
 __device__ __forceinline__ void wait_vm() { __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
+// Block barrier that orders LDS accesses only: unlike __syncthreads it does
+// not drain the wave's outstanding global loads, so prefetches issued before
+// it stay in flight across it.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 template <class L>
 __device__ __forceinline__ char* slab_ptr(const TableDev& t, uint64_t s) {
     return t.slabs + s * (uint64_t)L::SLAB_BYTES;

@@ -66,15 +66,15 @@ def _final(tbl, is_map, kb, vb):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("engine", [0, 1])
+@pytest.mark.parametrize("engine,combine", [(0, 2), (1, 1), (1, 2)])
 @pytest.mark.parametrize("lanes", [1, 2, 4, 8, 16, 32])
 @pytest.mark.parametrize("path", FIXTURES, ids=[os.path.basename(p) for p in FIXTURES])
-def test_gpu_reproduces_fixture(lslab, path, engine, lanes):
+def test_gpu_reproduces_fixture(lslab, path, engine, combine, lanes):
     import torch
     d = load(path)
-    print("[case] golden", d["name"], engine, lanes, flush=True)
+    print("[case] golden", d["name"], engine, combine, lanes, flush=True)
     is_map, kb, vb = bool(d["is_map"]), int(d["key_bytes"]), int(d["val_bytes"])
-    tun = lslab.tuning(lanes_per_op=lanes, engine=engine, max_subbatch=1 << 12)
+    tun = lslab.tuning(lanes_per_op=lanes, engine=engine, max_subbatch=1 << 12, combine=combine)
     tbl = (lslab.Map(kb, vb, int(d["n_log2"]), int(d["pool"]), tun) if is_map
            else lslab.Set(kb, int(d["n_log2"]), int(d["pool"]), tun))
     offs = d["offsets"]

@@ -42,8 +42,8 @@ def to_host(t, b):
     return a.view(np.uint32).astype(np.uint64) if b == 4 else a.view(np.uint64)
 
 
-def make(lslab, is_map, kb, vb, n_log2, pool, engine=0, lanes=16, subbatch=1 << 16):
-    tun = lslab.tuning(lanes_per_op=lanes, engine=engine, max_subbatch=subbatch)
+def make(lslab, is_map, kb, vb, n_log2, pool, engine=0, lanes=16, subbatch=1 << 16, combine=2):
+    tun = lslab.tuning(lanes_per_op=lanes, engine=engine, max_subbatch=subbatch, combine=combine)
     if is_map:
         return lslab.Map(kb, vb, n_log2, pool, tun)
     return lslab.Set(kb, n_log2, pool, tun)
@@ -179,15 +179,21 @@ def test_phase_separated_bit_exact(lslab, oracle_mod, is_map, kb, vb, engine, la
     assert st["max_chain"] >= 2
 
 
+# (engine, lanes per op, combining scope): the plain engine, and the two-phase
+# engine with wavefront-only and block-wide combining of same-key mutations.
+CONFLICT_MODES = [(0, 16, 2), (1, 16, 1), (1, 16, 2), (1, 1, 1), (1, 1, 2), (1, 4, 2)]
+
+
 @pytest.mark.parametrize("is_map,kb,vb", [(True, 8, 8), (True, 4, 4), (False, 8, 0)])
-@pytest.mark.parametrize("engine", ENGINES)
-def test_conflicting_batches_serial_order(lslab, oracle_mod, is_map, kb, vb, engine):
+@pytest.mark.parametrize("engine,lanes,combine", CONFLICT_MODES)
+def test_conflicting_batches_serial_order(lslab, oracle_mod, is_map, kb, vb, engine, lanes,
+                                          combine):
     """Zipf-like hot keys with get/put/remove conflicts inside each batch:
     every key's results must match some serial order of its ops."""
-    say("conflicting", is_map, kb, vb, engine)
+    say("conflicting", is_map, kb, vb, engine, lanes, combine)
     rng = np.random.default_rng(5 + engine)
     n_log2 = 5
-    tbl = make(lslab, is_map, kb, vb, n_log2, 1 << 14, engine, 16)
+    tbl = make(lslab, is_map, kb, vb, n_log2, 1 << 14, engine, lanes, combine=combine)
     ref = Replay(is_map, kb, vb, n_log2, pool_slabs=1 << 14)
     space = 3000
     p = 1.0 / np.arange(1, space + 1) ** 1.1
@@ -257,13 +263,14 @@ def test_plain_small_config(lslab, oracle_mod, engine):
         assert st["slabs_used"] == ref.slabs_used
 
 
+@pytest.mark.parametrize("combine", [1, 2])
 @pytest.mark.parametrize("is_map", [True, False])
-def test_exec_batches_uneven(lslab, oracle_mod, is_map):
+def test_exec_batches_uneven(lslab, oracle_mod, is_map, combine):
     """Uneven batches in one persistent launch; each batch checked in order."""
-    say("exec_batches", is_map)
+    say("exec_batches", is_map, combine)
     rng = np.random.default_rng(77)
     kb, vb = 8, 8
-    tbl = make(lslab, is_map, kb, vb, 6, 1 << 14, 1, 16, subbatch=1 << 12)
+    tbl = make(lslab, is_map, kb, vb, 6, 1 << 14, 1, 16, subbatch=1 << 12, combine=combine)
     ref = Replay(is_map, kb, vb, 6, pool_slabs=1 << 14)
     sizes = np.exp(rng.uniform(np.log(1), np.log(9000), size=40)).astype(np.int64)
     sizes[3] = 0  # an empty batch
