@@ -317,7 +317,8 @@ template <class L, int G, int BT, bool BC>
 __device__ __forceinline__ void phase_r_op(const TableDev& t, const OpsDev& o, const OutDev& out,
                                            uint64_t lo, uint32_t li, uint64_t m, bool act,
                                            uint8_t op, typename L::K key, typename L::V val,
-                                           uint64_t s, int slot, CombTab<BT>* ct) {
+                                           uint64_t s, int slot, CombTab<BT>* ct, RepEnt* rb,
+                                           uint32_t* rcnt) {
     using V = typename L::V;
     const int gl = Grp<G>::lane();
     const bool found = act && slot >= 0;
@@ -391,13 +392,61 @@ __device__ __forceinline__ void phase_r_op(const TableDev& t, const OpsDev& o, c
     fpred = __shfl(fpred, my_last, 64);
     if constexpr (BC) fdefer = __shfl((int)fdefer, my_last, 64) != 0;
     const uint32_t prev_li = __shfl(li, my_prev >= 0 ? my_prev : lane, 64);
-    if (lead) {
-        if (my_prev >= 0 || !fdefer)
-            mut_result<L>(t, o, out, lo, m, li, op, my_prev >= 0 ? prev_li + 1u : fpred, found,
-                          cur);
-        t.op_gidx[li] = g;
-        if (rep) t.op_loc[li] = found ? ((s << 5) | (uint64_t)slot) : NOLOC;
+    if (lead && (my_prev >= 0 || !fdefer))
+        mut_result<L>(t, o, out, lo, m, li, op, my_prev >= 0 ? prev_li + 1u : fpred, found, cur);
+    // The chain's last op may be the key's final mutation: append it to the
+    // block's list (one LDS atomic per wavefront) for phase W.
+    const bool cand = rep && g != 0xffffffffu;
+    const uint64_t cm = __ballot(cand);
+    if (cm) {
+        const int l0 = __builtin_ctzll(cm);
+        uint32_t wb = 0;
+        if (lane == l0)
+            wb = __hip_atomic_fetch_add(rcnt, (uint32_t)__builtin_popcountll(cm), __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_WORKGROUP);
+        wb = __builtin_amdgcn_readlane(wb, l0);
+        if (cand) {
+            RepEnt e;
+            e.li = li;
+            e.g = g;
+            e.loc = found ? ((s << 5) | (uint64_t)slot) : NOLOC;
+            rb[wb + __builtin_popcountll(cm & ((1ull << lane) - 1ull))] = e;
+        }
     }
+}
+
+// Start of block b's final-candidate list in a launch that runs one
+// sub-batch of m ops (k_phase_r / k_phase_w): the number of ops that blocks
+// 0..b-1 own when group gid (of ng, gpb per block) owns ops gid, gid + ng,
+// ...  Round r gives blocks 0..b-1 the ops [r*ng, r*ng + b*gpb) clipped to
+// m; blocks own disjoint ranges, so each block's list fits between its start
+// and the next block's, and all lists fit in m entries.
+__device__ __forceinline__ uint64_t block_rep_base(uint64_t m, uint64_t ng, uint64_t gpb,
+                                                   uint64_t b) {
+    const uint64_t A = b * gpb;
+    const uint64_t R = (m + ng - 1) / ng;
+    uint64_t F = 0;  // rounds in which all of blocks 0..b-1 have ops
+    if (m >= A) F = min(R, (m - A) / ng + 1);
+    return F * A + (R - F) * m - ng * ((R * (R - 1)) / 2 - (F * (F - 1)) / 2);
+}
+
+// Start of block b's final-candidate list in a persistent launch: a fixed
+// region of (ops a block can own in a sub-batch of submax ops) entries, the
+// same for every sub-batch.  The lists are written with plain stores, which
+// stay dirty in the writing XCD's L2 (grid barriers write nothing back);
+// with regions that moved with the sub-batch size, another XCD could later
+// write back a stale line over a newer entry.  Fixed regions mean each
+// entry address is only ever written from one CU.  All regions fit in
+// submax + ng entries.
+__device__ __forceinline__ uint64_t block_rep_base_fixed(uint64_t submax, uint64_t ng,
+                                                         uint64_t gpb, uint64_t b) {
+    return b * ((submax + ng - 1) / ng) * gpb;
+}
+
+// The block's LDS counter of final candidates.
+__device__ __forceinline__ uint32_t* rep_counter() {
+    __shared__ uint32_t cnt;
+    return &cnt;
 }
 
 // Phase R over ops [lo, lo + m) of a sub-batch, group gid of ng, as a
@@ -411,12 +460,13 @@ __device__ __forceinline__ void phase_r_op(const TableDev& t, const OpsDev& o, c
 template <class L, int G, int BT, bool BC>
 __device__ __forceinline__ void phase_r_range(const TableDev& t, const OpsDev& o,
                                               const OutDev& out, uint64_t lo, uint64_t m,
-                                              uint64_t gid, uint64_t ng) {
+                                              uint64_t gid, uint64_t ng, RepEnt* rb) {
     using K = typename L::K;
     using V = typename L::V;
     constexpr int S = Grp<G>::S;
     uint64_t rounds = 0;  // BC: rounds of the block's first group, the most of any
     CombTab<BT>* tabs = nullptr;
+    uint32_t* rcnt = rep_counter();
     if constexpr (BC) {
         const uint64_t gid0 = (uint64_t)blockIdx.x * blockDim.x / G;
         if (gid0 >= m) return;  // block-uniform
@@ -459,7 +509,7 @@ __device__ __forceinline__ void phase_r_range(const TableDev& t, const OpsDev& o
         for (int e = 0; e < S; ++e) blk1[e] = K(0);
         if (n1 < m) probe_issue<L, G, false>(t, (uint64_t)key1 & t.mask, blk1);
         phase_r_op<L, G, BT, BC>(t, o, out, lo, (uint32_t)li, m, !BC || li < m, op, key, val, s,
-                                 slot, BC ? tabs + (r & 1) : nullptr);
+                                 slot, BC ? tabs + (r & 1) : nullptr, rb, rcnt);
         if constexpr (!BC) {
             if (n1 >= m) break;
         }
@@ -476,33 +526,28 @@ __device__ __forceinline__ void phase_r_range(const TableDev& t, const OpsDev& o
     }
 }
 
-// Phase W of op (lo + li): the last mutation of each key writes its final
-// state and frees the combining-table entry.
+// Phase W of one final-mutation candidate: if it is still its key's last
+// mutation (nothing exchanged after it), it writes the key's final state and
+// frees the combining-table entry.
 template <class L, int G>
-__device__ __forceinline__ void phase_w_op(const TableDev& t, const OpsDev& o, uint64_t lo,
-                                           uint32_t li) {
+__device__ __forceinline__ void phase_w_ent(const TableDev& t, const OpsDev& o, uint64_t lo,
+                                            const RepEnt& e) {
     using K = typename L::K;
     using V = typename L::V;
     const int gl = Grp<G>::lane();
-    const uint64_t i = lo + li;
-    const uint8_t op = o.ops ? o.ops[i] : o.uop;
-    if (op == OP_GET) return;
-    const uint32_t g = t.op_gidx[li];
-    if (g == 0xffffffffu) return;
-    if (ld_agent(&t.gt_head[g]) != li + 1u) return;  // a later mutation of this key exists
-    const uint64_t loc = t.op_loc[li];
+    uint8_t op;
     K key;
     V val;
-    uint8_t op2;
-    load_op<L>(o, i, op2, key, val);
+    load_op<L>(o, lo + e.li, op, key, val);
+    if (ld_agent(&t.gt_head[e.g]) != e.li + 1u) return;  // a later mutation of this key exists
     if (gl == 0) {
-        __hip_atomic_store(&t.gt_head[g], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (g <= t.gt_mask)
-            __hip_atomic_store(&t.gt_key[g], GT_EMPTY, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&t.gt_head[e.g], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (e.g <= t.gt_mask)
+            __hip_atomic_store(&t.gt_key[e.g], GT_EMPTY, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    if (loc != NOLOC) {
-        const uint64_t s = loc >> 5;
-        const int slot = (int)(loc & 31u);
+    if (e.loc != NOLOC) {
+        const uint64_t s = e.loc >> 5;
+        const int slot = (int)(e.loc & 31u);
         if (gl == 0) {
             if (op == OP_PUT) {
                 if constexpr (L::IS_MAP) st_coh(val_slot<L>(t, s, slot), val);
@@ -516,6 +561,15 @@ __device__ __forceinline__ void phase_w_op(const TableDev& t, const OpsDev& o, u
         V rv;
         run_op<L, G, true>(t, OP_PUT, key, val, rf, rv, ok);
     }
+}
+
+// Phase W of a block: the cnt final-mutation candidates of its list rb, one
+// per group.
+template <class L, int G>
+__device__ __forceinline__ void phase_w_block(const TableDev& t, const OpsDev& o, uint64_t lo,
+                                              const RepEnt* rb, uint32_t cnt) {
+    const uint32_t gpb = blockDim.x / G;
+    for (uint32_t k = threadIdx.x / G; k < cnt; k += gpb) phase_w_ent<L, G>(t, o, lo, rb[k]);
 }
 
 }  // namespace lslab_dev

@@ -61,15 +61,24 @@ __global__ __launch_bounds__(256) void k_phase_r(TableDev t, OpsDev o, OutDev ou
                                                  uint64_t m) {
     const uint64_t ng = (uint64_t)gridDim.x * blockDim.x / G;
     const uint64_t gid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
-    if constexpr (BC) comb_init<256>();
-    phase_r_range<L, G, 256, BC>(t, o, out, lo, m, gid, ng);
+    uint32_t* rc = rep_counter();
+    if (threadIdx.x == 0) *rc = 0;
+    if constexpr (BC)
+        comb_init<256>();
+    else
+        __syncthreads();
+    phase_r_range<L, G, 256, BC>(t, o, out, lo, m, gid, ng,
+                                 t.rep + block_rep_base(m, ng, blockDim.x / G, blockIdx.x));
+    __syncthreads();
+    if (threadIdx.x == 0) t.rep_cnt[blockIdx.x] = *rc;
 }
 
+// Same grid as the k_phase_r launch before it: block b runs block b's list.
 template <class L, int G>
 __global__ __launch_bounds__(256) void k_phase_w(TableDev t, OpsDev o, uint64_t lo, uint64_t m) {
     const uint64_t ng = (uint64_t)gridDim.x * blockDim.x / G;
-    const uint64_t gid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
-    for (uint64_t li = gid; li < m; li += ng) phase_w_op<L, G>(t, o, lo, (uint32_t)li);
+    phase_w_block<L, G>(t, o, lo, t.rep + block_rep_base(m, ng, blockDim.x / G, blockIdx.x),
+                        t.rep_cnt[blockIdx.x]);
 }
 
 // Grid barrier for a grid that is fully resident.  Every wave waits for its
@@ -79,9 +88,13 @@ __global__ __launch_bounds__(256) void k_phase_w(TableDev t, OpsDev o, uint64_t 
 // stale cached lines) before the next phase reads the table with plain
 // loads.  No cache-wide write-back is issued.  Spins are bounded; on timeout
 // or abort the barrier returns false and the caller leaves the kernel.
-__device__ __forceinline__ bool grid_sync(BarState* bar, uint32_t nblocks) {
+// reset (optional): an LDS word that thread 0 zeroes while the whole block
+// is inside the barrier.
+__device__ __forceinline__ bool grid_sync(BarState* bar, uint32_t nblocks,
+                                          uint32_t* reset = nullptr) {
     wait_vm();
     __syncthreads();
+    if (reset && threadIdx.x == 0) *reset = 0;
     __shared__ uint32_t ok_s;
     if (threadIdx.x == 0) {
         uint32_t ok = 1;
@@ -136,20 +149,26 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_batches(TableDev t, OpsDev 
                                                  uint64_t submax, BarState* bar) {
     const uint64_t ng = (uint64_t)gridDim.x * blockDim.x / G;
     const uint64_t gid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
-    if constexpr (BC) comb_init<PERSIST_THREADS>();
+    uint32_t* rc = rep_counter();
+    RepEnt* rb = t.rep + block_rep_base_fixed(submax, ng, blockDim.x / G, blockIdx.x);
+    if (threadIdx.x == 0) *rc = 0;
+    if constexpr (BC)
+        comb_init<PERSIST_THREADS>();
+    else
+        __syncthreads();
     for (uint64_t b = 0; b < nbatches; ++b) {
         const uint64_t lo = offsets[b], hi = offsets[b + 1];
         for (uint64_t slo = lo; slo < hi; slo += submax) {
             const uint64_t m = min(submax, hi - slo);
-            phase_r_range<L, G, PERSIST_THREADS, BC>(t, o, out, slo, m, gid, ng);
+            phase_r_range<L, G, PERSIST_THREADS, BC>(t, o, out, slo, m, gid, ng, rb);
             if (!grid_sync(bar, gridDim.x)) {
                 if (threadIdx.x == 0)
                     __hip_atomic_fetch_or(t.status, ST_TIMEOUT, __ATOMIC_RELAXED,
                                           __HIP_MEMORY_SCOPE_AGENT);
                 return;
             }
-            for (uint64_t li = gid; li < m; li += ng) phase_w_op<L, G>(t, o, slo, (uint32_t)li);
-            if (!grid_sync(bar, gridDim.x)) {
+            phase_w_block<L, G>(t, o, slo, rb, *rc);
+            if (!grid_sync(bar, gridDim.x, rc)) {
                 if (threadIdx.x == 0)
                     __hip_atomic_fetch_or(t.status, ST_TIMEOUT, __ATOMIC_RELAXED,
                                           __HIP_MEMORY_SCOPE_AGENT);
@@ -256,7 +275,13 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_serve(TableDev t, OpsDev o,
     const uint64_t ng = (uint64_t)gridDim.x * blockDim.x / G;
     const uint64_t gid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
     __shared__ uint64_t stamp_s;
-    if constexpr (BC) comb_init<PERSIST_THREADS>();
+    uint32_t* rc = rep_counter();
+    RepEnt* rb = t.rep + block_rep_base_fixed(submax, ng, blockDim.x / G, blockIdx.x);
+    if (threadIdx.x == 0) *rc = 0;
+    if constexpr (BC)
+        comb_init<PERSIST_THREADS>();
+    else
+        __syncthreads();
     if (blockIdx.x == 0 && threadIdx.x == 0) ss->t0 = now_ticks();
     for (uint64_t tick = 0;; ++tick) {
         if (blockIdx.x == 0 && threadIdx.x == 0) serve_schedule(t, sv, ss);
@@ -267,10 +292,10 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_serve(TableDev t, OpsDev o,
             const uint64_t lo = sv.off[r_lo], hi = sv.off[r_hi];
             for (uint64_t slo = lo; slo < hi; slo += submax) {
                 const uint64_t m = min(submax, hi - slo);
-                phase_r_range<L, G, PERSIST_THREADS, BC>(t, o, out, slo, m, gid, ng);
+                phase_r_range<L, G, PERSIST_THREADS, BC>(t, o, out, slo, m, gid, ng, rb);
                 if (!grid_sync(bar, gridDim.x)) goto fail;
-                for (uint64_t li = gid; li < m; li += ng) phase_w_op<L, G>(t, o, slo, (uint32_t)li);
-                if (!grid_sync(bar, gridDim.x)) goto fail;
+                phase_w_block<L, G>(t, o, slo, rb, *rc);
+                if (!grid_sync(bar, gridDim.x, rc)) goto fail;
             }
             if (blockIdx.x == 0) {
                 if (threadIdx.x == 0) stamp_s = now_ticks() - ss->t0;

@@ -71,10 +71,21 @@ static void free_table(lslab_table* t) {
     (void)hipFree(t->state);
     (void)hipFree(t->dev.gt_key);
     (void)hipFree(t->dev.gt_head);
-    (void)hipFree(t->dev.op_gidx);
-    (void)hipFree(t->dev.op_loc);
+    (void)hipFree(t->dev.rep);
+    (void)hipFree(t->dev.rep_cnt);
     if (t->order_ev) (void)hipEventDestroy(t->order_ev);
     delete t;
+}
+
+// Blocks of a k_phase_r launch over up to submax ops: at least 2 groups per
+// block (64 threads, 32 lanes per op).
+static uint64_t rep_cnt_len(uint64_t submax) { return submax / 2 + 1; }
+
+// Final-candidate list entries: submax for a k_phase_r launch; up to submax
+// + (groups of the grid) for a persistent launch, whose blocks keep fixed
+// regions (one block of PERSIST_THREADS per CU, 1 lane per op at most).
+static uint64_t rep_len(const lslab_table* t) {
+    return t->tun.max_subbatch + (uint64_t)t->num_cus * PERSIST_THREADS;
 }
 
 static int alloc_scratch(lslab_table* t) {
@@ -82,16 +93,17 @@ static int alloc_scratch(lslab_table* t) {
     while (cap < 2 * t->tun.max_subbatch) cap <<= 1;
     (void)hipFree(t->dev.gt_key);
     (void)hipFree(t->dev.gt_head);
-    (void)hipFree(t->dev.op_gidx);
-    (void)hipFree(t->dev.op_loc);
+    (void)hipFree(t->dev.rep);
+    (void)hipFree(t->dev.rep_cnt);
     t->dev.gt_key = nullptr;
     t->dev.gt_head = nullptr;
-    t->dev.op_gidx = nullptr;
-    t->dev.op_loc = nullptr;
+    t->dev.rep = nullptr;
+    t->dev.rep_cnt = nullptr;
     if (hipMalloc(&t->dev.gt_key, (cap + 1) * sizeof(uint64_t)) != hipSuccess ||
         hipMalloc(&t->dev.gt_head, (cap + 1) * sizeof(uint32_t)) != hipSuccess ||
-        hipMalloc(&t->dev.op_gidx, t->tun.max_subbatch * sizeof(uint32_t)) != hipSuccess ||
-        hipMalloc(&t->dev.op_loc, t->tun.max_subbatch * sizeof(uint64_t)) != hipSuccess)
+        hipMalloc(&t->dev.rep, rep_len(t) * sizeof(RepEnt)) != hipSuccess ||
+        hipMalloc(&t->dev.rep_cnt, rep_cnt_len(t->tun.max_subbatch) * sizeof(uint32_t)) !=
+            hipSuccess)
         return LSLAB_ERR_OOM;
     HIPCHK(hipMemset(t->dev.gt_key, 0xff, (cap + 1) * sizeof(uint64_t)));
     HIPCHK(hipMemset(t->dev.gt_head, 0, (cap + 1) * sizeof(uint32_t)));
@@ -165,7 +177,7 @@ static int create_common(lslab_table** out, TableType type, bool is_map, int kb,
     t->dev.pool = pool;
     t->dev.mask = (uint32_t)(t->nbuckets - 1);
     t->device_bytes = slab_total + t->nbuckets * 4 + sizeof(DevState) + (t->gt_cap + 1) * 12 +
-                      t->submax * 12;
+                      rep_len(t) * sizeof(RepEnt) + rep_cnt_len(t->submax) * 4;
     *out = t;
     return LSLAB_OK;
 }

@@ -45,6 +45,15 @@ struct Layout {
 };
 
 // Device view of one table (passed by value into kernels).
+// A two-phase mutation that may be its key's final one in the sub-batch (the
+// last op of a combining chain): its op index, its key's combining-table
+// entry and the key's pre-batch location (slab << 5 | slot, or NOLOC).
+struct RepEnt {
+    uint32_t li;
+    uint32_t g;
+    uint64_t loc;
+};
+
 struct TableDev {
     char* slabs;           // (nbuckets + pool) * SLAB_BYTES, 256-B aligned
     uint32_t* locks;       // one RW lock word per bucket
@@ -52,8 +61,8 @@ struct TableDev {
     uint32_t* status;      // sticky ST_* bits
     uint64_t* gt_key;      // batch combining table: key per entry (GT_EMPTY = free)
     uint32_t* gt_head;     // last mutation (sub-batch op index + 1) per entry
-    uint32_t* op_gidx;     // per op: combining-table entry of its key
-    uint64_t* op_loc;      // per op: (slab << 5 | slot) of its key before the batch, or NOLOC
+    RepEnt* rep;           // final-mutation candidates, one list per block (see RepEnt)
+    uint32_t* rep_cnt;     // per block of the two-phase launch pair: its list length
     uint64_t nbuckets;
     uint64_t pool;
     uint32_t mask;         // nbuckets - 1

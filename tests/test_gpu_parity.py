@@ -310,3 +310,41 @@ def test_exec_batches_uneven(lslab, oracle_mod, is_map, combine):
         rk, rv = oracle_dump_sorted(ref)
         np.testing.assert_array_equal(k, rk)
         np.testing.assert_array_equal(vv, rv)
+
+
+@pytest.mark.parametrize("lanes,max_blocks", [(8, 0), (8, 128), (1, 0)])
+def test_exec_batches_many_sizes(lslab, oracle_mod, lanes, max_blocks):
+    """Thousands of uneven batches in one persistent launch: each block's
+    list of final-mutation candidates must stay intact while the sub-batch
+    size changes from batch to batch (a list overwritten by a stale copy
+    leaves a combining entry behind, seen as a wrong result or a CORRUPT
+    status)."""
+    say("exec_batches_many_sizes", lanes, max_blocks)
+    rng = np.random.default_rng(4242 + lanes + max_blocks)
+    kb = vb = 8
+    tbl = make(lslab, True, kb, vb, 12, 1 << 16, 1, lanes, subbatch=1 << 16, combine=1)
+    tbl.set_tuning(lslab.tuning(lanes_per_op=lanes, engine=1, max_blocks=max_blocks,
+                                max_subbatch=1 << 16, combine=1))
+    ref = Replay(True, kb, vb, 12, pool_slabs=1 << 16)
+    nb = 1500
+    sizes = np.exp(rng.uniform(np.log(1), np.log(40000), size=nb)).astype(np.int64)
+    n = int(sizes.sum())
+    space = 1 << 17
+    p = 1.0 / np.arange(1, space + 1) ** 0.99
+    p /= p.sum()
+    keys = rng.choice(space, size=n, p=p).astype(np.uint64) * np.uint64(0x9E3779B97F4A7C15)
+    ops = rng.choice([0, 1, 2], size=n, p=[0.7, 0.25, 0.05]).astype(np.uint8)
+    vals = rng.integers(1, 2**63, size=n, dtype=np.uint64)
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    f, v = tbl.exec_batches(torch.from_numpy(offs).cuda(), torch.from_numpy(ops).cuda(),
+                            to_dev(keys, kb), to_dev(vals, vb))
+    f, v = f.cpu().numpy(), to_host(v.contiguous(), vb)
+    assert tbl.status() == 0
+    for b in range(nb):
+        lo, hi = offs[b], offs[b + 1]
+        bad, first = ref.check_batch(ops[lo:hi], keys[lo:hi], vals[lo:hi], f[lo:hi], v[lo:hi])
+        assert bad == 0, (b, first)
+    k, vv = dump_sorted(tbl, True, kb, vb)
+    rk, rv = oracle_dump_sorted(ref)
+    np.testing.assert_array_equal(k, rk)
+    np.testing.assert_array_equal(vv, rv)
