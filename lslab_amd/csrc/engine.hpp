@@ -318,7 +318,7 @@ __device__ __forceinline__ void phase_r_op(const TableDev& t, const OpsDev& o, c
                                            uint64_t lo, uint32_t li, uint64_t m, bool act,
                                            uint8_t op, typename L::K key, typename L::V val,
                                            uint64_t s, int slot, CombTab<BT>* ct, RepEnt* rb,
-                                           uint32_t* rcnt) {
+                                           uint32_t* rcnt, RepEnt* lrep, uint32_t lcap) {
     using V = typename L::V;
     const int gl = Grp<G>::lane();
     const bool found = act && slot >= 0;
@@ -410,7 +410,11 @@ __device__ __forceinline__ void phase_r_op(const TableDev& t, const OpsDev& o, c
             e.li = li;
             e.g = g;
             e.loc = found ? ((s << 5) | (uint64_t)slot) : NOLOC;
-            rb[wb + __builtin_popcountll(cm & ((1ull << lane) - 1ull))] = e;
+            const uint32_t k = wb + (uint32_t)__builtin_popcountll(cm & ((1ull << lane) - 1ull));
+            if (k < lcap)
+                lrep[k] = e;
+            else
+                rb[k] = e;
         }
     }
 }
@@ -443,6 +447,15 @@ __device__ __forceinline__ uint64_t block_rep_base_fixed(uint64_t submax, uint64
     return b * ((submax + ng - 1) / ng) * gpb;
 }
 
+// Final candidates a persistent block keeps in LDS; more spill to its
+// global region (same index).
+constexpr uint32_t REP_LDS = 1024;
+
+__device__ __forceinline__ RepEnt* rep_lds() {
+    __shared__ RepEnt ents[REP_LDS];
+    return ents;
+}
+
 // The block's LDS counter of final candidates.
 __device__ __forceinline__ uint32_t* rep_counter() {
     __shared__ uint32_t cnt;
@@ -460,7 +473,8 @@ __device__ __forceinline__ uint32_t* rep_counter() {
 template <class L, int G, int BT, bool BC>
 __device__ __forceinline__ void phase_r_range(const TableDev& t, const OpsDev& o,
                                               const OutDev& out, uint64_t lo, uint64_t m,
-                                              uint64_t gid, uint64_t ng, RepEnt* rb) {
+                                              uint64_t gid, uint64_t ng, RepEnt* rb,
+                                              RepEnt* lrep = nullptr, uint32_t lcap = 0) {
     using K = typename L::K;
     using V = typename L::V;
     constexpr int S = Grp<G>::S;
@@ -509,7 +523,7 @@ __device__ __forceinline__ void phase_r_range(const TableDev& t, const OpsDev& o
         for (int e = 0; e < S; ++e) blk1[e] = K(0);
         if (n1 < m) probe_issue<L, G, false>(t, (uint64_t)key1 & t.mask, blk1);
         phase_r_op<L, G, BT, BC>(t, o, out, lo, (uint32_t)li, m, !BC || li < m, op, key, val, s,
-                                 slot, BC ? tabs + (r & 1) : nullptr, rb, rcnt);
+                                 slot, BC ? tabs + (r & 1) : nullptr, rb, rcnt, lrep, lcap);
         if constexpr (!BC) {
             if (n1 >= m) break;
         }
@@ -563,13 +577,15 @@ __device__ __forceinline__ void phase_w_ent(const TableDev& t, const OpsDev& o, 
     }
 }
 
-// Phase W of a block: the cnt final-mutation candidates of its list rb, one
-// per group.
+// Phase W of a block: the cnt final-mutation candidates of its list (the
+// first lcap in LDS at lrep, the rest at rb), one per group.
 template <class L, int G>
 __device__ __forceinline__ void phase_w_block(const TableDev& t, const OpsDev& o, uint64_t lo,
-                                              const RepEnt* rb, uint32_t cnt) {
+                                              const RepEnt* rb, uint32_t cnt,
+                                              const RepEnt* lrep = nullptr, uint32_t lcap = 0) {
     const uint32_t gpb = blockDim.x / G;
-    for (uint32_t k = threadIdx.x / G; k < cnt; k += gpb) phase_w_ent<L, G>(t, o, lo, rb[k]);
+    for (uint32_t k = threadIdx.x / G; k < cnt; k += gpb)
+        phase_w_ent<L, G>(t, o, lo, k < lcap ? lrep[k] : rb[k]);
 }
 
 }  // namespace lslab_dev

@@ -22,12 +22,15 @@ constexpr int PERSIST_THREADS = 1024;
 // Grid-wide barrier state for the persistent kernels.  Blocks arrive on one
 // of BAR_GROUPS group counters (block id mod BAR_GROUPS), the last block of a
 // group arrives on the top counter, and the last group releases every group
-// by bumping the group's generation word.  Each word sits on its own 128-B
-// line, so arrivals and polls of different groups do not share a line.
+// by bumping the group's generation word.  Counters only grow within a
+// launch (k_reset_bar zeroes them before it): a block's arrival count tells
+// it which barrier instance it is in, so nothing is reset on the critical
+// path and no generation is read before arriving.  Each word sits on its own
+// 128-B line, so arrivals and polls of different groups do not share a line.
 constexpr uint32_t BAR_GROUPS = 16;
 struct alignas(128) BarWord {
-    uint32_t count;
-    uint32_t gen;
+    uint32_t count;  // arrivals so far in this launch
+    uint32_t gen;    // completed barrier instances (group words only)
     uint32_t pad[30];
 };
 struct BarState {
@@ -84,13 +87,15 @@ __global__ __launch_bounds__(256) void k_phase_w(TableDev t, OpsDev o, uint64_t 
 // Grid barrier for a grid that is fully resident.  Every wave waits for its
 // own writes to complete (shared table writes are agent-scope stores or
 // atomics, so nothing else is needed to publish them), one lane per block
-// arrives, the last arrival releases everyone, and every wave acquires (drops
-// stale cached lines) before the next phase reads the table with plain
-// loads.  No cache-wide write-back is issued.  Spins are bounded; on timeout
-// or abort the barrier returns false and the caller leaves the kernel.
-// reset (optional): an LDS word that thread 0 zeroes while the whole block
-// is inside the barrier.
-__device__ __forceinline__ bool grid_sync(BarState* bar, uint32_t nblocks,
+// arrives, and the last arrival releases everyone.  With `acquire` the block
+// then drops stale cached lines (one L2 + L1 invalidate by wave 0) so that
+// the next phase may read the table with plain loads; a phase that reads
+// shared state only through agent-scope loads, or data its own CU wrote,
+// passes acquire = false.  No cache-wide write-back is issued.  Spins are
+// bounded; on timeout or abort the barrier returns false and the caller
+// leaves the kernel.  reset (optional): an LDS word that thread 0 zeroes
+// while the whole block is inside the barrier.
+__device__ __forceinline__ bool grid_sync(BarState* bar, uint32_t nblocks, bool acquire,
                                           uint32_t* reset = nullptr) {
     wait_vm();
     __syncthreads();
@@ -102,24 +107,20 @@ __device__ __forceinline__ bool grid_sync(BarState* bar, uint32_t nblocks,
         const uint32_t ngroups = nblocks < BAR_GROUPS ? nblocks : BAR_GROUPS;
         const uint32_t members = (nblocks - x + BAR_GROUPS - 1) / BAR_GROUPS;
         BarWord* gw = &bar->group[x];
-        const uint32_t g = ld_agent(&gw->gen);
         const uint32_t arrived =
             __hip_atomic_fetch_add(&gw->count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (arrived == members - 1u) {
-            st_coh(&gw->count, 0u);
-            wait_vm();
+        const uint32_t inst = arrived / members;  // this barrier's instance for the group
+        if ((arrived + 1u) % members == 0u) {
             const uint32_t top = __hip_atomic_fetch_add(&bar->top.count, 1u, __ATOMIC_RELAXED,
                                                         __HIP_MEMORY_SCOPE_AGENT);
-            if (top == ngroups - 1u) {
-                st_coh(&bar->top.count, 0u);
-                wait_vm();
+            if ((top + 1u) % ngroups == 0u) {
                 for (uint32_t y = 0; y < ngroups; ++y)
                     __hip_atomic_fetch_add(&bar->group[y].gen, 1u, __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_AGENT);
             }
         }
         const uint64_t t0 = now_ticks();
-        while (ld_agent(&gw->gen) == g) {
+        while ((int32_t)(ld_agent(&gw->gen) - inst) <= 0) {
             if (ld_agent(&bar->abort) || now_ticks() - t0 > SPIN_LIMIT_TICKS) {
                 st_coh(&bar->abort, 1u);
                 ok = 0;
@@ -133,7 +134,7 @@ __device__ __forceinline__ bool grid_sync(BarState* bar, uint32_t nblocks,
     // L2 and waits for it; the block barrier then releases the other waves.
     // (An acquire in every wave costs 4-8x more per barrier: each is a
     // cache-wide invalidate.)
-    if (threadIdx.x < 64) {
+    if (acquire && threadIdx.x < 64) {
         acquire_agent();
         wait_vm();
     }
@@ -160,15 +161,16 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_batches(TableDev t, OpsDev 
         const uint64_t lo = offsets[b], hi = offsets[b + 1];
         for (uint64_t slo = lo; slo < hi; slo += submax) {
             const uint64_t m = min(submax, hi - slo);
-            phase_r_range<L, G, PERSIST_THREADS, BC>(t, o, out, slo, m, gid, ng, rb);
-            if (!grid_sync(bar, gridDim.x)) {
+            phase_r_range<L, G, PERSIST_THREADS, BC>(t, o, out, slo, m, gid, ng, rb, rep_lds(),
+                                                     REP_LDS);
+            if (!grid_sync(bar, gridDim.x, false)) {
                 if (threadIdx.x == 0)
                     __hip_atomic_fetch_or(t.status, ST_TIMEOUT, __ATOMIC_RELAXED,
                                           __HIP_MEMORY_SCOPE_AGENT);
                 return;
             }
-            phase_w_block<L, G>(t, o, slo, rb, *rc);
-            if (!grid_sync(bar, gridDim.x, rc)) {
+            phase_w_block<L, G>(t, o, slo, rb, *rc, rep_lds(), REP_LDS);
+            if (!grid_sync(bar, gridDim.x, true, rc)) {
                 if (threadIdx.x == 0)
                     __hip_atomic_fetch_or(t.status, ST_TIMEOUT, __ATOMIC_RELAXED,
                                           __HIP_MEMORY_SCOPE_AGENT);
@@ -182,6 +184,7 @@ __device__ __forceinline__ void reset_bar(BarState* bar) {
     if (threadIdx.x <= BAR_GROUPS && blockIdx.x == 0) {
         BarWord* w = threadIdx.x == 0 ? &bar->top : &bar->group[threadIdx.x - 1];
         w->count = 0;
+        w->gen = 0;
         if (threadIdx.x == 0) bar->abort = 0;
     }
 }
@@ -285,17 +288,18 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_serve(TableDev t, OpsDev o,
     if (blockIdx.x == 0 && threadIdx.x == 0) ss->t0 = now_ticks();
     for (uint64_t tick = 0;; ++tick) {
         if (blockIdx.x == 0 && threadIdx.x == 0) serve_schedule(t, sv, ss);
-        if (!grid_sync(bar, gridDim.x)) goto fail;
+        if (!grid_sync(bar, gridDim.x, false)) goto fail;
         {
             if (ld_agent(&ss->done)) break;
             const uint64_t r_lo = ld_coh(&ss->r_lo), r_hi = ld_coh(&ss->r_hi);
             const uint64_t lo = sv.off[r_lo], hi = sv.off[r_hi];
             for (uint64_t slo = lo; slo < hi; slo += submax) {
                 const uint64_t m = min(submax, hi - slo);
-                phase_r_range<L, G, PERSIST_THREADS, BC>(t, o, out, slo, m, gid, ng, rb);
-                if (!grid_sync(bar, gridDim.x)) goto fail;
-                phase_w_block<L, G>(t, o, slo, rb, *rc);
-                if (!grid_sync(bar, gridDim.x, rc)) goto fail;
+                phase_r_range<L, G, PERSIST_THREADS, BC>(t, o, out, slo, m, gid, ng, rb, rep_lds(),
+                                                     REP_LDS);
+                if (!grid_sync(bar, gridDim.x, false)) goto fail;
+                phase_w_block<L, G>(t, o, slo, rb, *rc, rep_lds(), REP_LDS);
+                if (!grid_sync(bar, gridDim.x, true, rc)) goto fail;
             }
             if (blockIdx.x == 0) {
                 if (threadIdx.x == 0) stamp_s = now_ticks() - ss->t0;
