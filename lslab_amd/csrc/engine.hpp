@@ -96,7 +96,7 @@ __device__ __forceinline__ void apply_op(const TableDev& t, uint8_t op, typename
             const uint64_t idx = basei + __builtin_popcountll(am & ((1ull << leader) - 1ull));
             if (idx < t.pool) {
                 ins_s = t.nbuckets + idx;
-                ins_slot = 0;
+                ins_slot = L::SLOT0;
             } else {
                 ins = false;
                 ok = false;
@@ -477,7 +477,7 @@ __device__ __forceinline__ void phase_r_range(const TableDev& t, const OpsDev& o
                                               RepEnt* lrep = nullptr, uint32_t lcap = 0) {
     using K = typename L::K;
     using V = typename L::V;
-    constexpr int S = Grp<G>::S;
+    constexpr int S = PW<L, G>::N;
     uint64_t rounds = 0;  // BC: rounds of the block's first group, the most of any
     CombTab<BT>* tabs = nullptr;
     uint32_t* rcnt = rep_counter();

@@ -372,7 +372,7 @@ __global__ void k_dump(TableDev t, typename L::K* keys, typename L::V* vals, uin
     uint64_t s = b, hops = 0;
     for (;;) {
         const uint32_t v = *valid_word<L>(t, s) & L::CAPMASK;
-        for (int i = 0; i < L::CAP; ++i) {
+        for (int i = 0; i < 32; ++i) {
             if ((v >> i) & 1u) {
                 unsigned long long idx = atomicAdd(counter, 1ull);
                 if (idx < cap) {

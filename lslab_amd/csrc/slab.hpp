@@ -2,16 +2,21 @@
 // gfx950.
 //
 // Slab layout (one slab = one key block, then one value block for maps):
-//   key block: 32 slots of K.  Slots 0..CAP-1 hold keys.
-//     K = u64        : slot 31 = valid bitmap (low 32 bits) | next slab (high 32)
+//   key block: 32 slots of K.  Slots SLOT0..SLOT0+CAP-1 hold keys.
+//     K = u64        : slot 0 = valid bitmap (low 32 bits) | next slab (high 32),
+//                      keys in slots 1..31
 //     K = u32, map   : slot 31 = valid bitmap; next slab lives in value slot 31
 //     K = u32, set   : slot 30 = next slab, slot 31 = valid bitmap (CAP = 30)
 //   value block (maps): 32 slots of V; value slot i belongs to key slot i.
+//   Valid bit i belongs to key slot i.
 // This keeps the reference's 31 keys per slab and its valid-bit semantics
 // (detail/slab_node.h:13-35) with a 32-bit slab index in place of the 8-byte
 // pointer, so a u64 key block is exactly 256 B (two 128-B lines) and a u32 key
-// block one line.  Slabs 0..2^n-1 are the bucket heads; pool slabs follow them
-// in the same array, so next == 0 means end of chain.
+// block one line.  The u64 meta word sits in the first line together with
+// the first 15 keys: keys fill the lowest free slot first, so a probe of a
+// slab holding at most 15 keys, or of a key among its first 15, reads one
+// line (see PW).  Slabs 0..2^n-1 are the bucket heads; pool slabs follow
+// them in the same array, so next == 0 means end of chain.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -37,10 +42,11 @@ struct Layout {
     static constexpr uint32_t VB = MAP ? 32u * sizeof(V) : 0u;
     static constexpr uint32_t SLAB_BYTES = KB + VB;
     static constexpr int CAP = (sizeof(K) == 4 && !MAP) ? 30 : 31;
-    static constexpr uint32_t CAPMASK = (1u << CAP) - 1u;
+    static constexpr int SLOT0 = (sizeof(K) == 8) ? 1 : 0;  // first key slot
+    static constexpr uint32_t CAPMASK = ((1u << CAP) - 1u) << SLOT0;  // key slots
     // Byte offsets inside a slab.
-    static constexpr uint32_t VALID_OFF = (sizeof(K) == 8) ? 31u * 8u : 31u * 4u;
-    static constexpr uint32_t NEXT_OFF = (sizeof(K) == 8) ? 31u * 8u + 4u
+    static constexpr uint32_t VALID_OFF = (sizeof(K) == 8) ? 0u : 31u * 4u;
+    static constexpr uint32_t NEXT_OFF = (sizeof(K) == 8) ? 4u
                                          : (MAP ? KB + 31u * sizeof(V) : 30u * 4u);
 };
 
@@ -108,6 +114,17 @@ struct Grp {
     __device__ static uint64_t bits(uint64_t ballot) {
         return (ballot >> base()) & ((1ull << G) - 1ull);
     }
+};
+
+// Key slots one probe load covers, per G-lane group: the whole key block, or
+// for u64 keys (G <= 16) its first 128-B line, slots 0..15 (meta word and the
+// first 15 keys).  The second line is loaded only when the first cannot
+// settle the lookup: no match there and a valid bit set in slots 16..31.
+template <class L, int G>
+struct PW {
+    static constexpr bool SPLIT = sizeof(typename L::K) == 8 && G <= 16;
+    static constexpr int SPAN = SPLIT ? 16 : 32;  // slots per load
+    static constexpr int N = SPAN / G;            // slots per lane per load
 };
 
 template <typename T>
@@ -215,52 +232,66 @@ struct Probe {
 };
 
 // Group-cooperative probe of slab s for key.  Lane l of the group loads key
-// slots [l*S, l*S+S) in one coalesced access (a u64 key block is two 128-B
-// lines read by the whole group at once).  Plain loads when the caller
-// guarantees no concurrent writer (read-only launches, phase R); coherent
-// loads (COH) under a bucket lock.
-// Issue half of a probe: lane l of the group loads key slots [l*S, l*S+S).
+// slots [l*N, l*N+N) of the first load span in one coalesced access (PW).
+// Plain loads when the caller guarantees no concurrent writer (read-only
+// launches, phase R); coherent loads (COH) under a bucket lock.
+// probe_issue starts the load; probe_compute finishes the probe (loading the
+// second line of a split u64 block when needed).
 template <class L, int G, bool COH>
 __device__ __forceinline__ void probe_issue(const TableDev& t, uint64_t s,
-                                            typename L::K (&mine)[Grp<G>::S]) {
+                                            typename L::K (&mine)[PW<L, G>::N]) {
     using K = typename L::K;
-    constexpr int S = Grp<G>::S;
+    constexpr int N = PW<L, G>::N;
     const K* kb = reinterpret_cast<const K*>(slab_ptr<L>(t, s));
-    if constexpr (COH) load_slots_coherent<K, S>(kb + Grp<G>::lane() * S, mine);
-    else load_slots<K, S>(kb + Grp<G>::lane() * S, mine);
+    if constexpr (COH) load_slots_coherent<K, N>(kb + Grp<G>::lane() * N, mine);
+    else load_slots<K, N>(kb + Grp<G>::lane() * N, mine);
 }
 
-// Compute half of a probe over the loaded slots (group-uniform result).
+// Slot of key among the loaded slots [base + gl*N, base + gl*N + N), or -1
+// (group-uniform).
+template <class L, int G>
+__device__ __forceinline__ int probe_match(uint32_t valid, typename L::K key, int base,
+                                           const typename L::K (&mine)[PW<L, G>::N]) {
+    constexpr int N = PW<L, G>::N;
+    const uint32_t v = valid & L::CAPMASK;
+    int hit = -1;
+#pragma unroll
+    for (int e = 0; e < N; ++e) {
+        const int slot = base + Grp<G>::lane() * N + e;
+        if (((v >> slot) & 1u) && mine[e] == key) hit = slot;
+    }
+    const uint64_t b = Grp<G>::bits(__ballot(hit >= 0));
+    return b ? grp_bcast(hit, __builtin_ctzll(b), G) : -1;
+}
+
 template <class L, int G, bool COH>
 __device__ __forceinline__ Probe probe_compute(const TableDev& t, uint64_t s, typename L::K key,
-                                               const typename L::K (&mine)[Grp<G>::S]) {
+                                               const typename L::K (&mine)[PW<L, G>::N]) {
     using K = typename L::K;
-    constexpr int S = Grp<G>::S;
-    const int gl = Grp<G>::lane();
+    constexpr int N = PW<L, G>::N;
     Probe p;
     if constexpr (sizeof(K) == 8) {
-        uint64_t meta = grp_bcast((uint64_t)mine[S - 1], G - 1, G);
+        const uint64_t meta = grp_bcast((uint64_t)mine[0], 0, G);  // slot 0, lane 0
         p.valid = (uint32_t)meta;
         p.next = (uint32_t)(meta >> 32);
     } else if constexpr (!L::IS_MAP) {
-        p.valid = grp_bcast((uint32_t)mine[31 % S], 31 / S, G);
-        p.next = grp_bcast((uint32_t)mine[30 % S], 30 / S, G);
+        p.valid = grp_bcast((uint32_t)mine[31 % N], 31 / N, G);
+        p.next = grp_bcast((uint32_t)mine[30 % N], 30 / N, G);
     } else {
-        p.valid = grp_bcast((uint32_t)mine[S - 1], G - 1, G);
+        p.valid = grp_bcast((uint32_t)mine[N - 1], G - 1, G);
         p.next = 0;
     }
-    int hit = -1;
-#pragma unroll
-    for (int e = 0; e < S; ++e) {
-        int slot = gl * S + e;
-        if (slot < L::CAP && ((p.valid >> slot) & 1u) && mine[e] == key) hit = slot;
+    p.slot = probe_match<L, G>(p.valid, key, 0, mine);
+    if constexpr (PW<L, G>::SPLIT) {
+        if (p.slot < 0 && (p.valid & L::CAPMASK & 0xffff0000u)) {
+            K hi[N];
+            const K* kb = reinterpret_cast<const K*>(slab_ptr<L>(t, s)) + 16;
+            if constexpr (COH) load_slots_coherent<K, N>(kb + Grp<G>::lane() * N, hi);
+            else load_slots<K, N>(kb + Grp<G>::lane() * N, hi);
+            p.slot = probe_match<L, G>(p.valid, key, 16, hi);
+        }
     }
-    uint64_t b = Grp<G>::bits(__ballot(hit >= 0));
-    if (b) {
-        int src = __builtin_ctzll(b);
-        p.slot = grp_bcast(hit, src, G);
-    } else {
-        p.slot = -1;
+    if (p.slot < 0) {
         if constexpr (sizeof(K) == 4 && L::IS_MAP)
             p.next = COH ? ld_coh(next_word<L>(t, s)) : *next_word<L>(t, s);
     }
@@ -269,7 +300,7 @@ __device__ __forceinline__ Probe probe_compute(const TableDev& t, uint64_t s, ty
 
 template <class L, int G, bool COH>
 __device__ __forceinline__ Probe probe_slab(const TableDev& t, uint64_t s, typename L::K key) {
-    typename L::K mine[Grp<G>::S];
+    typename L::K mine[PW<L, G>::N];
     probe_issue<L, G, COH>(t, s, mine);
     return probe_compute<L, G, COH>(t, s, key, mine);
 }
