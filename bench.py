@@ -295,6 +295,37 @@ def cpu_baseline(ops, keys, vals, nb_sample):
             "cpu": _cpu_model()}
 
 
+def cpu_baseline_mt(ops, keys, vals, nb_sample, threads):
+    """The multithreaded host baseline (sharded std::unordered_map, one
+    shared_mutex per shard, a barrier between batches) on the same prefill
+    and the same nb_sample batches."""
+    import numpy as np
+    from oracle.oracle import MtBaseline
+    mt = MtBaseline(threads, N_LOG2, True)
+    n = 1 << SPACE_LOG2
+    mt.reserve(n)
+    i = np.arange(n, dtype=np.uint64) ^ np.uint64(PREFILL_SEED)
+    with np.errstate(over="ignore"):
+        z = i ^ (i >> np.uint64(33))
+        z = z * np.uint64(0xFF51AFD7ED558CCD)
+        z ^= z >> np.uint64(33)
+        z = z * np.uint64(0xC4CEB9FE1A85EC53)
+        z ^= z >> np.uint64(33)
+    pre_s, _, _ = mt.run(np.array([0, n], np.uint64), np.full(n, 1, np.uint8),
+                         np.arange(n, dtype=np.uint64), z)
+    del z, i
+    m = nb_sample * BATCH
+    offs = np.arange(0, m + 1, BATCH, dtype=np.uint64)
+    dt, _, _ = mt.run(offs, ops[:m].cpu().numpy(), keys[:m].cpu().numpy().view(np.uint64),
+                      vals[:m].cpu().numpy().view(np.uint64))
+    mt.close()
+    return {"value": round(m / dt / 1e6, 3), "unit": "Mops/s", "cores": threads, "kind": "port",
+            "sample": f"{nb_sample} batches x 2^20 ops of the same stream after a 2^26-key "
+                      f"prefill ({pre_s:.1f} s, untimed), std::unordered_map in "
+                      f"{16 * threads} shards, {threads} threads, barrier between batches",
+            "cpu": _cpu_model()}
+
+
 def _cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -439,6 +470,10 @@ def main():
         log("cpu baseline ...")
         res["cpu_baseline"] = cpu_baseline(ops[W * BATCH:], keys[W * BATCH:], vals[W * BATCH:],
                                            args.cpu_batches)
+        log("multithreaded cpu baseline ...")
+        res["cpu_baseline_mt"] = cpu_baseline_mt(ops[W * BATCH:], keys[W * BATCH:],
+                                                 vals[W * BATCH:], 4 * args.cpu_batches,
+                                                 min(16, os.cpu_count() or 1))
     print(json.dumps(res), flush=True)
 
 

@@ -105,6 +105,13 @@ public:
         return s;
     }
 
+    /// hist[i] = buckets whose chain has i+1 slabs (the last bin also counts
+    /// longer chains); nbins <= LSLAB_CHAIN_BINS.
+    void chain_histogram(uint64_t* hist, unsigned nbins = LSLAB_CHAIN_BINS,
+                         hipStream_t stream = 0) {
+        check(lslab_chain_histogram(t_, hist, nbins, stream), "map::chain_histogram");
+    }
+
     void set_tuning(const lslab_tuning& tu) { check(lslab_set_tuning(t_, &tu), "set_tuning"); }
 
     lslab_table* handle() { return t_; }

@@ -82,6 +82,10 @@ public:
         check(lslab_stats_get(t_, &s, stream), "set::stats");
         return s;
     }
+    void chain_histogram(uint64_t* hist, unsigned nbins = LSLAB_CHAIN_BINS,
+                         hipStream_t stream = 0) {
+        check(lslab_chain_histogram(t_, hist, nbins, stream), "set::chain_histogram");
+    }
     lslab_table* handle() { return t_; }
 
 private:

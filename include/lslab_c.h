@@ -182,8 +182,14 @@ int lslab_status(lslab_table* t, hipStream_t stream);
 
 /* Introspection (synchronous).  dump writes up to cap present pairs into the
  * DEVICE arrays keys_out / vals_out (vals_out ignored for sets; order
- * unspecified) and the number of present keys into *count. */
+ * unspecified) and the number of present keys into *count.
+ * chain_histogram (new; the reference has no introspection) writes into the
+ * HOST array hist[0..nbins) the number of buckets whose chain has i+1 slabs,
+ * with the last bin counting every longer chain too; 1 <= nbins <=
+ * LSLAB_CHAIN_BINS. */
+#define LSLAB_CHAIN_BINS 16
 int lslab_stats_get(lslab_table* t, lslab_stats* out, hipStream_t stream);
+int lslab_chain_histogram(lslab_table* t, uint64_t* hist, unsigned nbins, hipStream_t stream);
 int lslab_dump(lslab_table* t, void* keys_out, void* vals_out, uint64_t cap, uint64_t* count,
                hipStream_t stream);
 

@@ -26,6 +26,8 @@ ENGINE_TWO_PHASE = 1
 COMBINE_WAVE = 1
 COMBINE_BLOCK = 2
 
+CHAIN_BINS = 16
+
 # Every symbol include/lslab_c.h declares.
 EXPORTS = (
     "lslab_map_create", "lslab_set_create", "lslab_destroy", "lslab_set_tuning",
@@ -33,6 +35,7 @@ EXPORTS = (
     "lslab_map_remove", "lslab_map_mixed", "lslab_set_insert", "lslab_set_contains",
     "lslab_set_remove", "lslab_set_mixed", "lslab_exec_batches", "lslab_status",
     "lslab_stats_get", "lslab_dump", "lslab_error_string", "lslab_serve", "lslab_serve_result",
+    "lslab_chain_histogram",
 )
 
 SERVE_REJECTED = 0xFFFFFFFF
@@ -124,6 +127,7 @@ def load():
         "lslab_dump": [vp, vp, vp, u64, ctypes.POINTER(ctypes.c_uint64), vp],
         "lslab_serve": [vp, ctypes.POINTER(ServeParams), vp, vp, vp, vp, vp],
         "lslab_serve_result": [vp, ctypes.POINTER(ServeStats), vp],
+        "lslab_chain_histogram": [vp, ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint, vp],
     }
     for name, args in sig.items():
         f = getattr(lib, name)

@@ -333,6 +333,7 @@ __global__ void k_reset_serve(ServeState* ss, BarState* bar) {
 struct StatsDev {
     unsigned long long present;
     unsigned long long max_chain;
+    unsigned long long hist[LSLAB_CHAIN_BINS];  // buckets by chain length (slabs), last bin open
 };
 
 template <class L>
@@ -353,6 +354,12 @@ __global__ void k_stats(TableDev t, StatsDev* st) {
             s = nx;
         }
     }
+    // Chain-length histogram: per block in LDS, then one atomic per bin.
+    __shared__ unsigned int hist_s[LSLAB_CHAIN_BINS];
+    if (threadIdx.x < LSLAB_CHAIN_BINS) hist_s[threadIdx.x] = 0;
+    __syncthreads();
+    if (b < t.nbuckets)
+        atomicAdd(&hist_s[len < LSLAB_CHAIN_BINS ? len - 1 : LSLAB_CHAIN_BINS - 1], 1u);
     // wave reduction then one atomic per wave
     for (int off = 32; off > 0; off >>= 1) {
         present += __shfl_down(present, off, 64);
@@ -362,6 +369,9 @@ __global__ void k_stats(TableDev t, StatsDev* st) {
         atomicAdd(&st->present, present);
         atomicMax(&st->max_chain, len);
     }
+    __syncthreads();
+    if (threadIdx.x < LSLAB_CHAIN_BINS && hist_s[threadIdx.x])
+        atomicAdd(&st->hist[threadIdx.x], (unsigned long long)hist_s[threadIdx.x]);
 }
 
 template <class L>

@@ -451,6 +451,24 @@ extern "C" int lslab_stats_get(lslab_table* t, lslab_stats* out, hipStream_t str
     return LSLAB_OK;
 }
 
+extern "C" int lslab_chain_histogram(lslab_table* t, uint64_t* hist, unsigned nbins,
+                                     hipStream_t stream) {
+    if (!t || !hist || nbins < 1 || nbins > LSLAB_CHAIN_BINS) return LSLAB_ERR_INVALID;
+    HIPCHK(hipStreamWaitEvent(stream, t->order_ev, 0));
+    HIPCHK(hipMemsetAsync(&t->state->stats, 0, sizeof(StatsDev), stream));
+    int rc = dispatch(t, [&](auto l) {
+        return Launch<decltype(l)>::stats(ctx(t), &t->state->stats, stream);
+    });
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(stream));
+    StatsDev hs;
+    HIPCHK(hipMemcpy(&hs, &t->state->stats, sizeof(hs), hipMemcpyDeviceToHost));
+    for (unsigned i = 0; i < nbins; ++i) hist[i] = hs.hist[i];
+    for (unsigned i = nbins; i < LSLAB_CHAIN_BINS; ++i) hist[nbins - 1] += hs.hist[i];
+    HIPCHK(hipEventRecord(t->order_ev, stream));
+    return LSLAB_OK;
+}
+
 extern "C" int lslab_dump(lslab_table* t, void* keys_out, void* vals_out, uint64_t cap,
                           uint64_t* count, hipStream_t stream) {
     if (!t || !count || (cap && !keys_out)) return LSLAB_ERR_INVALID;

@@ -91,6 +91,14 @@ class _Table:
         _lib.check(_lib.load().lslab_stats_get(self._h, ctypes.byref(st), _stream(stream)), "stats")
         return {f: getattr(st, f) for f, _ in _lib.Stats._fields_}
 
+    def chain_histogram(self, nbins=_lib.CHAIN_BINS, stream=None):
+        """Buckets by chain length: entry i counts chains of i+1 slabs (the last
+        entry also counts longer chains)."""
+        h = (ctypes.c_uint64 * nbins)()
+        _lib.check(_lib.load().lslab_chain_histogram(self._h, h, nbins, _stream(stream)),
+                   "chain_histogram")
+        return [int(x) for x in h]
+
     def buckets(self):
         return 1 << self.n_log2
 

@@ -206,6 +206,18 @@ uint64_t orc_max_chain(const orc_table* t) {
     return best;
 }
 
+void orc_chain_hist(const orc_table* t, uint64_t* hist, unsigned nbins) {
+    for (unsigned i = 0; i < nbins; ++i) hist[i] = 0;
+    for (uint64_t b = 0; b < t->nbuckets; ++b) {
+        uint64_t len = 1, s = b;
+        while (t->next[s]) {
+            s = t->next[s];
+            len++;
+        }
+        hist[len < nbins ? len - 1 : nbins - 1]++;
+    }
+}
+
 uint64_t orc_dump(const orc_table* t, uint64_t* keys, uint64_t* vals, uint64_t cap) {
     uint64_t c = 0;
     for (uint64_t b = 0; b < t->nbuckets; ++b) {

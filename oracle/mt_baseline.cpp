@@ -94,6 +94,12 @@ void* mtb_create(int threads, unsigned n_log2, int is_map) {
 
 void mtb_destroy(void* p) { delete static_cast<Mtb*>(p); }
 
+// Sizes every shard for `total` keys spread evenly (no rehash while filling).
+void mtb_reserve(void* p, uint64_t total) {
+    Mtb* h = static_cast<Mtb*>(p);
+    for (auto& s : h->shards) s->m.reserve(total / h->shards.size() + 1);
+}
+
 // Runs nbatches batches ([offsets[b], offsets[b+1])) and returns the wall time
 // in seconds.  found_out / val_out may be NULL.
 double mtb_run(void* p, uint64_t nbatches, const uint64_t* offsets, const uint8_t* ops,

@@ -45,6 +45,8 @@ def _lib():
     for f in ("orc_present", "orc_slabs_used", "orc_max_chain"):
         getattr(lib, f).restype = u64
         getattr(lib, f).argtypes = [vp]
+    lib.orc_chain_hist.restype = None
+    lib.orc_chain_hist.argtypes = [vp, u64p, ctypes.c_uint]
     lib.orc_dump.restype = u64
     lib.orc_dump.argtypes = [vp, u64p, u64p, u64]
     lib.orc_check_batch.restype = i64
@@ -106,6 +108,11 @@ class Replay:
     def slabs_used(self):
         return int(_lib().orc_slabs_used(self.h))
 
+    def chain_hist(self, nbins=16):
+        h = np.zeros(nbins, np.uint64)
+        _lib().orc_chain_hist(self.h, _p(h), nbins)
+        return [int(x) for x in h]
+
     @property
     def max_chain(self):
         return int(_lib().orc_max_chain(self.h))
@@ -135,3 +142,67 @@ class Replay:
         bad = _lib().orc_check_batch(self.h, n, _p(ops), _p(keys), _p(vals), _p(found),
                                      _p(got), _p(pf), _p(pv), ctypes.byref(fb))
         return int(bad), (None if fb.value == 2**64 - 1 else int(fb.value))
+
+
+_MT = None
+
+
+def _mt_lib():
+    global _MT
+    if _MT is not None:
+        return _MT
+    path = os.path.join(_HERE, "libmt_baseline.so")
+    if not os.path.exists(path):
+        build()
+    lib = ctypes.CDLL(path)
+    vp = ctypes.c_void_p
+    lib.mtb_create.restype = vp
+    lib.mtb_create.argtypes = [ctypes.c_int, ctypes.c_uint, ctypes.c_int]
+    lib.mtb_destroy.argtypes = [vp]
+    lib.mtb_run.restype = ctypes.c_double
+    lib.mtb_run.argtypes = [vp, ctypes.c_uint64, vp, vp, vp, vp, vp, vp]
+    lib.mtb_size.restype = ctypes.c_uint64
+    lib.mtb_size.argtypes = [vp]
+    lib.mtb_reserve.argtypes = [vp, ctypes.c_uint64]
+    _MT = lib
+    return lib
+
+
+class MtBaseline:
+    """The multithreaded host baseline (mt_baseline.cpp): std::unordered_map
+    shards by bucket bits, a shared_mutex per shard, T threads, a barrier
+    between batches.  Timing infrastructure, checked against Replay."""
+
+    def __init__(self, threads, n_log2, is_map=True):
+        self._lib = _mt_lib()
+        self.threads = threads
+        self._h = self._lib.mtb_create(threads, n_log2, 1 if is_map else 0)
+
+    def close(self):
+        if self._h:
+            self._lib.mtb_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def run(self, offsets, ops, keys, vals=None, want_results=False):
+        """Applies batches [offsets[b], offsets[b+1]); returns (seconds, found, vals)."""
+        offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+        n = int(offsets[-1])
+        ops = np.ascontiguousarray(ops, dtype=np.uint8)
+        keys = _u64(keys, n)
+        vals = None if vals is None else _u64(vals, n)
+        f = np.zeros(n, np.uint8) if want_results else None
+        o = np.zeros(n, np.uint64) if want_results else None
+        dt = self._lib.mtb_run(self._h, len(offsets) - 1, _p(offsets), _p(ops), _p(keys), _p(vals),
+                               _p(f), _p(o))
+        return dt, f, o
+
+    def reserve(self, total):
+        """Sizes the shards for `total` keys."""
+        self._lib.mtb_reserve(self._h, total)
+
+    @property
+    def size(self):
+        return int(self._lib.mtb_size(self._h))

@@ -348,3 +348,27 @@ def test_exec_batches_many_sizes(lslab, oracle_mod, lanes, max_blocks):
     rk, rv = oracle_dump_sorted(ref)
     np.testing.assert_array_equal(k, rk)
     np.testing.assert_array_equal(vv, rv)
+
+
+@pytest.mark.parametrize("is_map,kb,vb", [(True, 8, 8), (True, 4, 4), (False, 8, 0),
+                                         (False, 4, 0)])
+def test_chain_histogram_matches_oracle(lslab, oracle_mod, is_map, kb, vb):
+    """Insert-only table: the device chain-length histogram equals the
+    oracle's (chains are then determined by the per-bucket key counts)."""
+    say("chain_histogram", is_map, kb, vb)
+    rng = np.random.default_rng(17)
+    n_log2 = 8
+    tbl = make(lslab, is_map, kb, vb, n_log2, 1 << 12)
+    cap = 30 if (not is_map and kb == 4) else 31  # keys per slab of the layout
+    ref = Replay(is_map, kb, vb, n_log2, pool_slabs=1 << 12, slab_keys=cap)
+    # skewed bucket loads: bucket b gets about b/4 keys, some chains span many slabs
+    keys = np.concatenate([np.arange(rng.integers(0, 300), dtype=np.uint64) * np.uint64(256)
+                           + np.uint64(b) for b in range(256)])
+    rng.shuffle(keys)
+    ops = np.full(len(keys), OP_PUT, np.uint8)
+    run_batch(tbl, is_map, kb, vb, ops, keys, keys)
+    ref.apply(ops, keys, keys if is_map else None)
+    assert tbl.status() == 0
+    assert tbl.chain_histogram(16) == ref.chain_hist(16)
+    assert tbl.chain_histogram(3) == ref.chain_hist(3)
+    assert tbl.stats()["max_chain"] == ref.max_chain

@@ -127,6 +127,21 @@ def test_slab_accounting_closed_forms(oracle_mod):
     assert r.slabs_used == 2  # reused the last slab's hole
 
 
+def test_chain_histogram_closed_form(oracle_mod):
+    # Insert-only: bucket i holding c_i keys has a chain of max(1, ceil(c_i/31)) slabs.
+    counts = [0, 1, 31, 32, 62, 63, 200, 500]
+    r = Replay(True, 8, 8, 3, pool_slabs=100)
+    keys = np.concatenate([np.arange(c, dtype=np.uint64) * np.uint64(8) + np.uint64(i)
+                           for i, c in enumerate(counts)])
+    r.apply(np.full(len(keys), OP_PUT, np.uint8), keys, keys)
+    want = [0] * 16
+    for c in counts:
+        want[min(max(1, -(-c // 31)), 16) - 1] += 1
+    assert r.chain_hist(16) == want
+    short = r.chain_hist(4)  # the last bin collects the longer chains
+    assert short[:3] == want[:3] and short[3] == sum(want[3:])
+
+
 def test_put_on_reused_slot_returns_zero(oracle_mod):
     # A removed slot keeps its stale value; a later insert of another key there
     # must still report 0 as the previous value.
@@ -246,3 +261,33 @@ def test_checker_rejects_planted_mistakes(oracle_mod, is_map):
         # end state: insert then a true remove cannot end present
         assert small([OP_PUT, OP_REMOVE], [1, 1], post=1) == 1
         assert small([OP_PUT, OP_REMOVE], [1, 1], post=0) == 0
+
+
+@pytest.mark.parametrize("is_map", [True, False])
+def test_mt_baseline_matches_replay(oracle_mod, is_map):
+    """The multithreaded host baseline (timed beside the sequential replay in
+    bench.py) gives the replay's results op by op on batches whose ops touch
+    distinct keys, with batch order kept by its barrier."""
+    from oracle.oracle import MtBaseline
+    rng = np.random.default_rng(91 + is_map)
+    nb, space = 12, 6000
+    sizes = rng.integers(1, 3000, size=nb)
+    keys = np.concatenate([rng.permutation(space)[:s] for s in sizes]).astype(np.uint64)
+    keys = keys * np.uint64(0x9E3779B97F4A7C15)
+    n = len(keys)
+    ops = rng.choice([0, 1, 2], size=n, p=[0.4, 0.45, 0.15]).astype(np.uint8)
+    vals = rng.integers(1, 2**63, size=n, dtype=np.uint64)
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    mt = MtBaseline(4, 10, is_map)
+    dt, f, v = mt.run(offs, ops, keys, vals if is_map else None, want_results=True)
+    assert dt > 0
+    r = Replay(is_map, 8, 8 if is_map else 0, 10, pool_slabs=1 << 12)
+    for b in range(nb):
+        sl = slice(int(offs[b]), int(offs[b + 1]))
+        rf, rv, fail = r.apply(ops[sl], keys[sl], vals[sl] if is_map else None)
+        assert fail == 0
+        np.testing.assert_array_equal(f[sl], rf, err_msg=f"batch {b}")
+        if is_map:
+            np.testing.assert_array_equal(v[sl], rv, err_msg=f"batch {b}")
+    assert mt.size == r.present
+    mt.close()
