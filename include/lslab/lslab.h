@@ -30,13 +30,41 @@ inline void check(int rc, const char* what = "") {
     if (rc != LSLAB_OK) throw error(rc, what);
 }
 
-/// Identity hash: the reference's hash<T> (include/lslab/hash.h:10-15).  The
-/// device side always buckets by (size_t)key & (2^n - 1); this type exists so
-/// that code naming lslab::hash<K> keeps compiling.
+/// Identity hash: the reference's hash<T> (include/lslab/hash.h:10-15).  A
+/// table created with it buckets by (size_t)key & (2^n - 1) on the device.
 template <typename T>
 struct hash {
+    static constexpr int kind = LSLAB_HASH_IDENTITY;
     size_t operator()(T x) const { return static_cast<size_t>(x); }
 };
+
+/// Mixing hash (new): buckets by the murmur3 64-bit finalizer of the key, for
+/// keys whose low bits are not uniform.  The device computes it; this host
+/// functor gives the same value.
+template <typename T>
+struct mix_hash {
+    static constexpr int kind = LSLAB_HASH_MIX;
+    size_t operator()(T x) const {
+        uint64_t z = static_cast<uint64_t>(x);
+        z ^= z >> 33;
+        z *= 0xff51afd7ed558ccdull;
+        z ^= z >> 33;
+        z *= 0xc4ceb9fe1a85ec53ull;
+        z ^= z >> 33;
+        return static_cast<size_t>(z);
+    }
+};
+
+namespace detail {
+/// The caller's tuning (or the defaults) with the hash of the Hash functor.
+template <typename Hash>
+inline lslab_tuning with_hash(const lslab_tuning* tuning) {
+    lslab_tuning tu{};
+    if (tuning) tu = *tuning;
+    tu.hash = Hash::kind;
+    return tu;
+}
+}  // namespace detail
 
 /// Layout of cuda::std::pair<bool, V>: the reference's get() result
 /// (include/lslab/map.h:208-211).

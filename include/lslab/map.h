@@ -28,7 +28,8 @@ public:
     explicit map(unsigned n_log_2, uint64_t pool_slabs = 10000,
                  const lslab_tuning* tuning = nullptr)
         : n_log_2_(n_log_2) {
-        check(lslab_map_create(&t_, sizeof(K), sizeof(V), n_log_2, pool_slabs, tuning),
+        const lslab_tuning tu = detail::with_hash<Hash>(tuning);
+        check(lslab_map_create(&t_, sizeof(K), sizeof(V), n_log_2, pool_slabs, &tu),
               "map_create");
     }
 

@@ -23,7 +23,8 @@ public:
     explicit set(unsigned n_log_2, uint64_t pool_slabs = 10000,
                  const lslab_tuning* tuning = nullptr)
         : n_log_2_(n_log_2) {
-        check(lslab_set_create(&t_, sizeof(K), n_log_2, pool_slabs, tuning), "set_create");
+        const lslab_tuning tu = detail::with_hash<Hash>(tuning);
+        check(lslab_set_create(&t_, sizeof(K), n_log_2, pool_slabs, &tu), "set_create");
     }
 
     set(const set&) = delete;

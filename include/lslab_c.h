@@ -61,6 +61,13 @@ extern "C" {
 #define LSLAB_COMBINE_WAVE 1
 #define LSLAB_COMBINE_BLOCK 2
 
+/* Bucket hash, fixed when the table is created.  IDENTITY is the reference's
+ * hash<T> (include/lslab/hash.h:10-15): bucket = key & (2^n - 1).  MIX buckets
+ * by the murmur3 64-bit finalizer of the key (zero-extended for 32-bit keys),
+ * for key sets whose low bits are not uniform. */
+#define LSLAB_HASH_IDENTITY 1
+#define LSLAB_HASH_MIX 2
+
 typedef struct lslab_table lslab_table;
 
 typedef struct {
@@ -71,6 +78,8 @@ typedef struct {
     uint64_t max_subbatch; /* two-phase ops per sub-batch; sizes the combining
                               table and per-op scratch (0 = 2^20) */
     int combine;           /* LSLAB_COMBINE_* for the two-phase engine (0 = BLOCK) */
+    int hash;              /* LSLAB_HASH_* at create (0 = IDENTITY); set_tuning keeps it
+                              (0 there means "unchanged", another value is an error) */
 } lslab_tuning;
 
 typedef struct {

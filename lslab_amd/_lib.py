@@ -26,6 +26,9 @@ ENGINE_TWO_PHASE = 1
 COMBINE_WAVE = 1
 COMBINE_BLOCK = 2
 
+HASH_IDENTITY = 1
+HASH_MIX = 2
+
 CHAIN_BINS = 16
 
 # Every symbol include/lslab_c.h declares.
@@ -49,6 +52,7 @@ class Tuning(ctypes.Structure):
         ("max_blocks", ctypes.c_int),
         ("max_subbatch", ctypes.c_uint64),
         ("combine", ctypes.c_int),
+        ("hash", ctypes.c_int),
     ]
 
 

@@ -128,7 +128,7 @@ template <class L, int G, bool LOCKED>
 __device__ __forceinline__ void run_op(const TableDev& t, uint8_t op, typename L::K key,
                                        typename L::V val, bool& rfound, typename L::V& rval,
                                        bool& ok) {
-    const uint64_t b = (uint64_t)key & t.mask;  // identity hash, masked (hash.h:10-15)
+    const uint64_t b = bucket_of(t, (uint64_t)key);
     if constexpr (!LOCKED) {
         apply_op<L, G, false>(t, op, key, val, b, rfound, rval, ok);
         return;
@@ -497,7 +497,7 @@ __device__ __forceinline__ void phase_r_range(const TableDev& t, const OpsDev& o
     K blk[S];
 #pragma unroll
     for (int e = 0; e < S; ++e) blk[e] = K(0);
-    if (!BC || li < m) probe_issue<L, G, false>(t, (uint64_t)key & t.mask, blk);
+    if (!BC || li < m) probe_issue<L, G, false>(t, bucket_of(t, (uint64_t)key), blk);
     uint64_t n1 = li + ng;
     uint8_t op1 = OP_GET;
     K key1 = K(0);
@@ -512,7 +512,7 @@ __device__ __forceinline__ void phase_r_range(const TableDev& t, const OpsDev& o
         K key2 = K(0);
         V val2 = V(0);
         if (n2 < m) load_op<L>(o, lo + n2, op2, key2, val2);
-        const uint64_t b = (uint64_t)key & t.mask;
+        const uint64_t b = bucket_of(t, (uint64_t)key);
         uint64_t s = b;
         int slot = -1;
         uint32_t valid;
@@ -521,7 +521,7 @@ __device__ __forceinline__ void phase_r_range(const TableDev& t, const OpsDev& o
         K blk1[S];
 #pragma unroll
         for (int e = 0; e < S; ++e) blk1[e] = K(0);
-        if (n1 < m) probe_issue<L, G, false>(t, (uint64_t)key1 & t.mask, blk1);
+        if (n1 < m) probe_issue<L, G, false>(t, bucket_of(t, (uint64_t)key1), blk1);
         phase_r_op<L, G, BT, BC>(t, o, out, lo, (uint32_t)li, m, !BC || li < m, op, key, val, s,
                                  slot, BC ? tabs + (r & 1) : nullptr, rb, rcnt, lrep, lcap);
         if constexpr (!BC) {

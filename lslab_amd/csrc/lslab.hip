@@ -51,6 +51,7 @@ static void normalise(lslab_tuning& tu) {
     if (tu.block_threads == 0) tu.block_threads = 256;
     if (tu.max_subbatch == 0) tu.max_subbatch = 1ull << 20;
     if (tu.combine == 0) tu.combine = LSLAB_COMBINE_BLOCK;
+    if (tu.hash == 0) tu.hash = LSLAB_HASH_IDENTITY;
 }
 
 static bool tuning_ok(const lslab_tuning& tu) {
@@ -61,6 +62,7 @@ static bool tuning_ok(const lslab_tuning& tu) {
     if (tu.max_blocks < 0) return false;
     if (tu.max_subbatch < 64 || tu.max_subbatch > (1ull << 30)) return false;
     if (tu.combine != LSLAB_COMBINE_WAVE && tu.combine != LSLAB_COMBINE_BLOCK) return false;
+    if (tu.hash != LSLAB_HASH_IDENTITY && tu.hash != LSLAB_HASH_MIX) return false;
     return true;
 }
 
@@ -176,6 +178,7 @@ static int create_common(lslab_table** out, TableType type, bool is_map, int kb,
     t->dev.nbuckets = t->nbuckets;
     t->dev.pool = pool;
     t->dev.mask = (uint32_t)(t->nbuckets - 1);
+    t->dev.hash_mix = t->tun.hash == LSLAB_HASH_MIX;
     t->device_bytes = slab_total + t->nbuckets * 4 + sizeof(DevState) + (t->gt_cap + 1) * 12 +
                       rep_len(t) * sizeof(RepEnt) + rep_cnt_len(t->submax) * 4;
     *out = t;
@@ -212,8 +215,10 @@ extern "C" int lslab_destroy(lslab_table* t) {
 extern "C" int lslab_set_tuning(lslab_table* t, const lslab_tuning* tuning) {
     if (!t || !tuning) return LSLAB_ERR_INVALID;
     lslab_tuning tu = *tuning;
+    if (tu.hash == 0) tu.hash = t->tun.hash;  // 0 keeps the table's hash
     normalise(tu);
     if (!tuning_ok(tu)) return LSLAB_ERR_INVALID;
+    if (tu.hash != t->tun.hash) return LSLAB_ERR_INVALID;  // the table's buckets depend on it
     const bool resize = tu.max_subbatch != t->tun.max_subbatch;
     t->tun = tu;
     if (resize) {

@@ -73,6 +73,7 @@ struct TableDev {
     uint64_t pool;
     uint32_t mask;         // nbuckets - 1
     uint32_t gt_mask;      // gt capacity - 1
+    uint32_t hash_mix;     // bucket by mix64(key) instead of the key itself
 };
 
 // Operation stream: per-op opcode (or a uniform one), keys and values with a
@@ -103,6 +104,13 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
     z *= 0xc4ceb9fe1a85ec53ull;
     z ^= z >> 33;
     return z;
+}
+
+// Bucket of a key: the reference's identity hash masked to 2^n buckets
+// (include/lslab/hash.h:10-15, map.h:95-96), or the murmur3 finalizer of
+// the key when the table was created with LSLAB_HASH_MIX.
+__device__ __forceinline__ uint64_t bucket_of(const TableDev& t, uint64_t key) {
+    return (t.hash_mix ? mix64(key) : key) & t.mask;
 }
 
 template <int G>

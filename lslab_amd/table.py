@@ -43,8 +43,9 @@ def _ops(t, n):
 
 
 def tuning(lanes_per_op=4, engine=_lib.ENGINE_PLAIN, block_threads=256, max_blocks=0,
-           max_subbatch=1 << 20, combine=_lib.COMBINE_BLOCK):
-    return _lib.Tuning(lanes_per_op, engine, block_threads, max_blocks, max_subbatch, combine)
+           max_subbatch=1 << 20, combine=_lib.COMBINE_BLOCK, hash=_lib.HASH_IDENTITY):
+    return _lib.Tuning(lanes_per_op, engine, block_threads, max_blocks, max_subbatch, combine,
+                       hash)
 
 
 class _Table:

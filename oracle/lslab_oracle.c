@@ -35,6 +35,7 @@ struct orc_table {
     unsigned n_log2;
     uint64_t nbuckets, pool, used, present;
     uint64_t key_mask, val_mask;
+    int hash_mix;     /* bucket by fmix64(key) instead of the key (LSLAB_HASH_MIX) */
     uint64_t* keys;   /* (nbuckets + pool) * 32 */
     uint64_t* vals;   /* maps only, same shape */
     uint32_t* valid;  /* per slab */
@@ -78,11 +79,29 @@ void orc_destroy(orc_table* t) {
     free(t);
 }
 
+/* murmur3's 64-bit finalizer (Appleby, MurmurHash3 fmix64). */
+static uint64_t fmix64(uint64_t z) {
+    z ^= z >> 33;
+    z *= 0xff51afd7ed558ccdull;
+    z ^= z >> 33;
+    z *= 0xc4ceb9fe1a85ec53ull;
+    z ^= z >> 33;
+    return z;
+}
+
+void orc_set_hash(orc_table* t, int mix) { t->hash_mix = mix != 0; }
+
+/* Bucket of a key: the reference's identity hash (hash.h:10-15) masked to
+ * the table, or fmix64 of the key for a MIX table. */
+static uint64_t bucket(const orc_table* t, uint64_t key) {
+    return (t->hash_mix ? fmix64(key) : key) & (t->nbuckets - 1);
+}
+
 /* Find key: returns 1 and (slab, slot) when present.  Also reports the last
  * slab of the chain for a subsequent insert. */
 static int find(const orc_table* t, uint64_t key, uint64_t* slab_out, int* slot_out,
                 uint64_t* last_out) {
-    uint64_t s = key & (t->nbuckets - 1);
+    uint64_t s = bucket(t, key);
     for (;;) {
         uint32_t v = t->valid[s];
         const uint64_t* k = t->keys + s * 32;

@@ -51,6 +51,9 @@ int orc_lookup(const orc_table* t, uint64_t key, uint64_t* val_out);
 uint64_t orc_present(const orc_table* t);
 uint64_t orc_slabs_used(const orc_table* t);
 uint64_t orc_max_chain(const orc_table* t);
+/* Buckets by fmix64(key) (mix != 0) instead of the reference's identity
+ * hash; call before the first apply. */
+void orc_set_hash(orc_table* t, int mix);
 /* Buckets by chain length in slabs: hist[i] = chains of i+1 slabs, the last
  * bin also counting longer ones. */
 void orc_chain_hist(const orc_table* t, uint64_t* hist, unsigned nbins);

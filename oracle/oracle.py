@@ -45,6 +45,8 @@ def _lib():
     for f in ("orc_present", "orc_slabs_used", "orc_max_chain"):
         getattr(lib, f).restype = u64
         getattr(lib, f).argtypes = [vp]
+    lib.orc_set_hash.restype = None
+    lib.orc_set_hash.argtypes = [vp, ctypes.c_int]
     lib.orc_chain_hist.restype = None
     lib.orc_chain_hist.argtypes = [vp, u64p, ctypes.c_uint]
     lib.orc_dump.restype = u64
@@ -69,12 +71,15 @@ def _u64(a, n):
 class Replay:
     """Sequential replay table: map (is_map=True) or set."""
 
-    def __init__(self, is_map, key_bytes, val_bytes, n_log2, pool_slabs=10000, slab_keys=31):
+    def __init__(self, is_map, key_bytes, val_bytes, n_log2, pool_slabs=10000, slab_keys=31,
+                 mix_hash=False):
         self.is_map = bool(is_map)
         self.h = _lib().orc_create(int(is_map), key_bytes, val_bytes if is_map else 0,
                                    n_log2, pool_slabs, slab_keys)
         if not self.h:
             raise ValueError("orc_create rejected the configuration")
+        if mix_hash:
+            _lib().orc_set_hash(self.h, 1)
 
     def close(self):
         if self.h:

@@ -3,6 +3,8 @@
 //   GetPutTestGroupAllocator   test/slabunified_test.cu:156-205 (a pool of a
 //                              different size stands in for the foreign allocator)
 //   ContainsInsertRemoveTest   test/set_test.cu:65-106
+// GetPutTest also runs on a table created with lslab::mix_hash (the
+// reference's Hash template parameter with a non-identity functor).
 // Exit code 0 = all passed.
 #include <hip/hip_runtime.h>
 
@@ -42,11 +44,12 @@ struct dvec {
     }
 };
 
+template <typename Hash = lslab::hash<int>>
 static void get_put(uint64_t pool, int engine) {
     constexpr int size = 1024;
     lslab_tuning tu{};
     tu.engine = engine;
-    lslab::map<int, int> m(10, pool, &tu);
+    lslab::map<int, int, Hash> m(10, pool, &tu);
     std::vector<int> hk(size);
     for (int i = 0; i < size; ++i) hk[i] = i + 1;  // thrust::sequence(.., 1)
     dvec<int> keys(size), values(size), res1(size);
@@ -101,6 +104,8 @@ int main() {
             std::printf("GetPutTestGroupAllocator engine %d ok\n", engine);
             contains_insert_remove(engine);
             std::printf("ContainsInsertRemoveTest engine %d ok\n", engine);
+            get_put<lslab::mix_hash<int>>(10000, engine);
+            std::printf("GetPutTest mix_hash engine %d ok\n", engine);
         }
     } catch (const std::exception& e) {
         std::printf("FAILED: %s\n", e.what());

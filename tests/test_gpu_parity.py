@@ -372,3 +372,44 @@ def test_chain_histogram_matches_oracle(lslab, oracle_mod, is_map, kb, vb):
     assert tbl.chain_histogram(16) == ref.chain_hist(16)
     assert tbl.chain_histogram(3) == ref.chain_hist(3)
     assert tbl.stats()["max_chain"] == ref.max_chain
+
+
+@pytest.mark.parametrize("is_map,kb,vb", [(True, 8, 8), (False, 4, 0)])
+@pytest.mark.parametrize("engine", ENGINES)
+def test_mix_hash_table(lslab, oracle_mod, is_map, kb, vb, engine):
+    """A table created with LSLAB_HASH_MIX: keys that share their low bits
+    (one bucket under the identity hash) spread over the buckets exactly as
+    the oracle's fmix64 places them, and every result is the replay's."""
+    say("mix_hash", is_map, kb, vb, engine)
+    n_log2 = 7
+    cap = 30 if (not is_map and kb == 4) else 31
+    tun = lslab.tuning(lanes_per_op=4, engine=engine, max_subbatch=1 << 14, hash=lslab.HASH_MIX)
+    tbl = (lslab.Map(kb, vb, n_log2, 1 << 12, tun) if is_map
+           else lslab.Set(kb, n_log2, 1 << 12, tun))
+    ref = Replay(is_map, kb, vb, n_log2, pool_slabs=1 << 12, slab_keys=cap, mix_hash=True)
+    keys = np.arange(1, 6001, dtype=np.uint64) << np.uint64(n_log2)
+    rng = np.random.default_rng(3)
+    rng.shuffle(keys)
+    for ops in (np.full(len(keys), OP_PUT, np.uint8),
+                np.zeros(len(keys), np.uint8),
+                rng.choice([OP_GET, OP_PUT, OP_REMOVE], size=len(keys)).astype(np.uint8)):
+        vals = rng.integers(1, 2**31, size=len(keys), dtype=np.uint64)
+        f, v = run_batch(tbl, is_map, kb, vb, ops, keys, vals)
+        rf, rv, fail = ref.apply(ops, keys, vals if is_map else None)
+        assert fail == 0
+        np.testing.assert_array_equal(f, rf)
+        np.testing.assert_array_equal(v, rv)
+        if (ops == OP_PUT).all():
+            assert tbl.chain_histogram(16) == ref.chain_hist(16)
+    assert tbl.status() == 0
+    k, vv = dump_sorted(tbl, is_map, kb, vb)
+    rk, rvv = oracle_dump_sorted(ref)
+    np.testing.assert_array_equal(k, rk)
+    np.testing.assert_array_equal(vv, rvv)
+    # the table's hash cannot change after creation; 0 keeps it
+    with pytest.raises(Exception):
+        tbl.set_tuning(lslab.tuning(lanes_per_op=4, engine=engine, hash=lslab.HASH_IDENTITY))
+    tbl.set_tuning(lslab.tuning(lanes_per_op=2, engine=engine, max_subbatch=1 << 14, hash=0))
+    f, _ = run_batch(tbl, is_map, kb, vb, np.zeros(len(keys), np.uint8), keys, keys)
+    rf, _, _ = ref.apply(np.zeros(len(keys), np.uint8), keys)
+    np.testing.assert_array_equal(f, rf)

@@ -291,3 +291,37 @@ def test_mt_baseline_matches_replay(oracle_mod, is_map):
             np.testing.assert_array_equal(v[sl], rv, err_msg=f"batch {b}")
     assert mt.size == r.present
     mt.close()
+
+
+def _fmix64(x):
+    x = np.asarray(x, np.uint64)
+    with np.errstate(over="ignore"):
+        x = x ^ (x >> np.uint64(33))
+        x = x * np.uint64(0xFF51AFD7ED558CCD)
+        x = x ^ (x >> np.uint64(33))
+        x = x * np.uint64(0xC4CEB9FE1A85EC53)
+        x = x ^ (x >> np.uint64(33))
+    return x
+
+
+def test_mix_hash_buckets_closed_form(oracle_mod):
+    """A MIX table buckets by murmur3's fmix64 (published constants): its chain
+    histogram equals the one computed from fmix64(key) & mask in numpy, and
+    results do not depend on the hash."""
+    n_log2 = 6
+    keys = np.arange(1, 3001, dtype=np.uint64) << np.uint64(n_log2)  # all in bucket 0 by identity
+    for mix in (False, True):
+        r = Replay(True, 8, 8, n_log2, pool_slabs=1000, mix_hash=mix)
+        f, v, fail = r.apply(np.full(len(keys), OP_PUT, np.uint8), keys, keys + np.uint64(5))
+        assert fail == 0 and not f.any()
+        f, v, _ = r.apply(np.zeros(len(keys), np.uint8), keys)
+        assert f.all()
+        np.testing.assert_array_equal(v, keys + np.uint64(5))
+        b = (_fmix64(keys) if mix else keys) & np.uint64((1 << n_log2) - 1)
+        counts = np.bincount(b.astype(np.int64), minlength=1 << n_log2)
+        want = [0] * 16
+        for c in counts:
+            want[min(max(1, -(-int(c) // 31)), 16) - 1] += 1
+        assert r.chain_hist(16) == want
+    # the identity table piles everything into one long chain; the mixed one does not
+    assert want[0] + want[1] + want[2] == 1 << n_log2
