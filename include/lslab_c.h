@@ -68,6 +68,17 @@ extern "C" {
 #define LSLAB_HASH_IDENTITY 1
 #define LSLAB_HASH_MIX 2
 
+/* Slot and slab reuse, fixed when the table is created.  OFF is the
+ * reference's policy (detail/traverse.h:54-55,98-124;
+ * detail/device_allocator.h:62-73): an insert uses only the last slab's
+ * empty slots, and pool slabs are never freed.  ON (new): an insert takes the
+ * lowest empty slot of the first slab of its chain that has one; a remove that
+ * leaves a pool slab empty unlinks and frees it; new slabs come from the
+ * freed ones before the bump pointer.  Long runs with removes then keep their
+ * pool headroom.  stats.slabs_used counts slabs in chains. */
+#define LSLAB_RECLAIM_OFF 1
+#define LSLAB_RECLAIM_ON 2
+
 typedef struct lslab_table lslab_table;
 
 typedef struct {
@@ -80,6 +91,7 @@ typedef struct {
     int combine;           /* LSLAB_COMBINE_* for the two-phase engine (0 = BLOCK) */
     int hash;              /* LSLAB_HASH_* at create (0 = IDENTITY); set_tuning keeps it
                               (0 there means "unchanged", another value is an error) */
+    int reclaim;           /* LSLAB_RECLAIM_* at create (0 = OFF); set_tuning keeps it */
 } lslab_tuning;
 
 typedef struct {

@@ -29,6 +29,9 @@ COMBINE_BLOCK = 2
 HASH_IDENTITY = 1
 HASH_MIX = 2
 
+RECLAIM_OFF = 1
+RECLAIM_ON = 2
+
 CHAIN_BINS = 16
 
 # Every symbol include/lslab_c.h declares.
@@ -53,6 +56,7 @@ class Tuning(ctypes.Structure):
         ("max_subbatch", ctypes.c_uint64),
         ("combine", ctypes.c_int),
         ("hash", ctypes.c_int),
+        ("reclaim", ctypes.c_int),
     ]
 
 

@@ -35,33 +35,122 @@ __device__ __forceinline__ void load_op(const OpsDev& o, uint64_t i, uint8_t& op
     val = (L::IS_MAP && o.vals && op == OP_PUT) ? ld_stride<V>(o.vals, o.vstride, i) : V(0);
 }
 
-// Applies one op to the chain of bucket b.  Must be called by all G lanes of
-// the group; when LOCKED the caller holds the bucket lock (shared for gets,
-// exclusive otherwise).  Allocation of new slabs is aggregated over all groups
-// of the wave that reach this point together: one atomicAdd per wave.
+// Free list of pool slabs (reclaim policy): a LIFO stack linked through the
+// slabs' next words, its head word tagged (tag << 32 | slab) so that a pop
+// racing with a pop-and-push of the same slab fails its compare-and-swap.
+// One lane each; spins are bounded (a lost race retries, a damaged list
+// turns into a status bit).
+template <class L>
+__device__ __forceinline__ void free_push(const TableDev& t, uint64_t s) {
+    for (uint32_t tries = 0; tries < (1u << 20); ++tries) {
+        uint64_t old = ld_agent(t.free_head);
+        st_coh(next_word<L>(t, s), (uint32_t)old);
+        wait_vm();
+        const uint64_t nw = (((old >> 32) + 1ull) << 32) | s;
+        if (__hip_atomic_compare_exchange_strong(t.free_head, &old, nw, __ATOMIC_RELAXED,
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            __hip_atomic_fetch_add(t.free_count, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return;
+        }
+    }
+    __hip_atomic_fetch_or(t.status, ST_TIMEOUT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Pops a free pool slab, or returns 0 (none free).
+template <class L>
+__device__ __forceinline__ uint64_t free_pop(const TableDev& t) {
+    for (uint32_t tries = 0; tries < (1u << 20); ++tries) {
+        uint64_t old = ld_agent(t.free_head);
+        const uint64_t s = (uint32_t)old;
+        if (s == 0) return 0;
+        if (s < t.nbuckets || s >= t.nbuckets + t.pool) {
+            __hip_atomic_fetch_or(t.status, ST_CORRUPT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return 0;
+        }
+        const uint64_t nx = ld_coh(next_word<L>(t, s));
+        const uint64_t nw = (((old >> 32) + 1ull) << 32) | nx;
+        if (__hip_atomic_compare_exchange_strong(t.free_head, &old, nw, __ATOMIC_RELAXED,
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            __hip_atomic_fetch_add(t.free_count, ~0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return s;
+        }
+    }
+    __hip_atomic_fetch_or(t.status, ST_TIMEOUT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return 0;
+}
+
+// Unlinks every empty pool slab of bucket b's chain and frees it (reclaim
+// policy).  The caller holds b's exclusive lock; all G lanes call it.
+template <class L, int G>
+__device__ __forceinline__ void compact_chain(const TableDev& t, uint64_t b) {
+    const int gl = Grp<G>::lane();
+    uint64_t pv = b;
+    uint64_t nx = ld_coh(next_word<L>(t, b));
+    for (uint64_t hops = 0; nx != 0; ++hops) {
+        if (nx < t.nbuckets || nx >= t.nbuckets + t.pool || hops > t.pool) {
+            if (gl == 0)
+                __hip_atomic_fetch_or(t.status, ST_CORRUPT, __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
+            return;
+        }
+        const uint32_t v = ld_coh(valid_word<L>(t, nx)) & L::CAPMASK;
+        const uint64_t nn = ld_coh(next_word<L>(t, nx));
+        if (v == 0) {
+            if (gl == 0) {
+                st_coh(next_word<L>(t, pv), (uint32_t)nn);
+                wait_vm();
+                free_push<L>(t, nx);
+            }
+        } else {
+            pv = nx;
+        }
+        nx = nn;
+    }
+}
+
+// Applies one op to bucket b's chain.  With COH the caller holds the bucket
+// lock (shared for gets, exclusive otherwise) and every slab access is
+// agent-coherent; without, loads are plain (read-only launches).  All G
+// lanes of the group call it; lane 0 performs the writes.
+// Allocation of new slabs is aggregated over all groups of the wave that
+// reach this point together: one atomicAdd per wave.
 // Returns (found, value) with the reference's meaning per op:
 //   get    -> (present, value)      put -> (was present, old value)
 //   remove -> (was present, value)  set insert -> (true, 0)
 // and ok=false if the pool was exhausted.
+// Placement: the reference's policy (the last slab's lowest empty slot, else
+// a new slab linked at the end), or under reclaim the first slab with an
+// empty slot, a freed slab before a fresh one, and removal of a pool slab
+// that a remove leaves empty.
 template <class L, int G, bool COH>
 __device__ __forceinline__ void apply_op(const TableDev& t, uint8_t op, typename L::K key,
                                          typename L::V val, uint64_t b, bool& rfound,
                                          typename L::V& rval, bool& ok) {
     using V = typename L::V;
     const int gl = Grp<G>::lane();
-    uint64_t s;
+    ok = true;
+    if (op == OP_COMPACT) {
+        if constexpr (COH) compact_chain<L, G>(t, b);
+        rfound = false;
+        rval = V(0);
+        return;
+    }
+    uint64_t s, prev = NOSLAB, room = NOSLAB;
     int slot;
-    uint32_t valid;
-    walk<L, G, COH>(t, b, key, s, slot, valid);
+    uint32_t valid, room_valid = 0;
+    const bool rc = COH && t.reclaim && op != OP_GET;
+    if (rc)
+        walk_full<L, G, COH>(t, b, key, s, slot, valid, prev, room, room_valid);
+    else
+        walk<L, G, COH>(t, b, key, s, slot, valid);
     const bool found = slot >= 0;
     V cur = V(0);
     if constexpr (L::IS_MAP) {
         if (found) cur = COH ? ld_coh(val_slot<L>(t, s, slot)) : *val_slot<L>(t, s, slot);
     }
-    ok = true;
     rfound = found;
     rval = cur;
-    bool ins = false, need_alloc = false;
+    bool ins = false, need_alloc = false, fresh = false;
     int ins_slot = -1;
     uint64_t ins_s = s;
     if (op == OP_PUT) {
@@ -71,19 +160,50 @@ __device__ __forceinline__ void apply_op(const TableDev& t, uint8_t op, typename
             }
         } else {
             ins = true;
-            uint32_t empty = ~valid & L::CAPMASK;
+            uint32_t tv = valid;
+            if (rc && room != NOSLAB) {
+                ins_s = room;
+                tv = room_valid;
+            }
+            const uint32_t empty = ~tv & L::CAPMASK;
             if (empty) ins_slot = __builtin_ctz(empty);
             else need_alloc = true;
         }
         if constexpr (!L::IS_MAP) rfound = true;
     } else if (op == OP_REMOVE) {
-        if (found && gl == 0)
-            __hip_atomic_fetch_and(valid_word<L>(t, s), ~(1u << slot), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+        if (found) {
+            uint32_t old = 0;
+            if (gl == 0)
+                old = __hip_atomic_fetch_and(valid_word<L>(t, s), ~(1u << slot), __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+            if (rc) {
+                old = grp_bcast(old, 0, G);
+                // Under the lock: a pool slab left empty leaves its chain.
+                if (s >= t.nbuckets && prev != NOSLAB &&
+                    ((old & ~(1u << slot)) & L::CAPMASK) == 0 && gl == 0) {
+                    st_coh(next_word<L>(t, prev), ld_coh(next_word<L>(t, s)));
+                    wait_vm();
+                    free_push<L>(t, s);
+                }
+            }
+        }
+    }
+    // Reclaim: a freed slab before a fresh one (one pop per group).
+    bool need_bump = need_alloc;
+    if (rc && need_alloc) {
+        uint64_t got = 0;
+        if (gl == 0) got = free_pop<L>(t);
+        got = grp_bcast(got, 0, G);
+        if (got) {
+            ins_s = got;
+            ins_slot = L::SLOT0;
+            fresh = true;
+            need_bump = false;
+        }
     }
     // Wave-aggregated bump allocation: one atomic for every group that needs
     // a fresh slab in this step.
-    const uint64_t am = __ballot(need_alloc && gl == 0);
+    const uint64_t am = __ballot(need_bump && gl == 0);
     if (am) {
         const int first = __builtin_ctzll(am);
         uint64_t basei = 0;
@@ -91,12 +211,13 @@ __device__ __forceinline__ void apply_op(const TableDev& t, uint8_t op, typename
             basei = __hip_atomic_fetch_add(t.alloc_ctr, (uint64_t)__builtin_popcountll(am),
                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         basei = grp_bcast(basei, first, 64);
-        if (need_alloc) {
+        if (need_bump) {
             const int leader = Grp<G>::base();
             const uint64_t idx = basei + __builtin_popcountll(am & ((1ull << leader) - 1ull));
             if (idx < t.pool) {
                 ins_s = t.nbuckets + idx;
                 ins_slot = L::SLOT0;
+                fresh = true;
             } else {
                 ins = false;
                 ok = false;
@@ -108,14 +229,16 @@ __device__ __forceinline__ void apply_op(const TableDev& t, uint8_t op, typename
         }
     }
     if (ins && gl == 0) {
-        // Key and value first, then the valid bit, then the link: each step
+        // A reused slab still carries its free-list link: clear it first.
+        // Key and value, then the valid bit, then the link: each step
         // completes before the next is issued.
+        if (fresh && t.reclaim) st_coh(next_word<L>(t, ins_s), 0u);
         st_coh(key_slot<L>(t, ins_s, ins_slot), key);
         if constexpr (L::IS_MAP) st_coh(val_slot<L>(t, ins_s, ins_slot), val);
         wait_vm();
         __hip_atomic_fetch_or(valid_word<L>(t, ins_s), 1u << ins_slot, __ATOMIC_RELAXED,
                               __HIP_MEMORY_SCOPE_AGENT);
-        if (ins_s != s) {
+        if (fresh) {
             wait_vm();
             st_coh(next_word<L>(t, s), (uint32_t)ins_s);
         }
@@ -562,12 +685,24 @@ __device__ __forceinline__ void phase_w_ent(const TableDev& t, const OpsDev& o, 
     if (e.loc != NOLOC) {
         const uint64_t s = e.loc >> 5;
         const int slot = (int)(e.loc & 31u);
+        uint32_t old = 0;
         if (gl == 0) {
             if (op == OP_PUT) {
                 if constexpr (L::IS_MAP) st_coh(val_slot<L>(t, s, slot), val);
             } else {
-                __hip_atomic_fetch_and(valid_word<L>(t, s), ~(1u << slot), __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
+                old = __hip_atomic_fetch_and(valid_word<L>(t, s), ~(1u << slot),
+                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        // Reclaim: this remove left a pool slab empty; unlink it (and any
+        // other empty pool slab of the chain) under the bucket's lock, which
+        // also orders it after any insert into the same chain.
+        if (t.reclaim && op != OP_PUT && s >= t.nbuckets) {
+            old = grp_bcast(old, 0, G);
+            if (((old & ~(1u << slot)) & L::CAPMASK) == 0) {
+                bool rf, ok;
+                V rv;
+                run_op<L, G, true>(t, OP_COMPACT, key, V(0), rf, rv, ok);
             }
         }
     } else if (op == OP_PUT) {

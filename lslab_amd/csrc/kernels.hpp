@@ -235,7 +235,7 @@ constexpr uint32_t SERVE_REJECTED = 0xffffffffu;
 
 __device__ inline void serve_schedule(const TableDev& t, const ServeDev& sv, ServeState* ss) {
     uint64_t r = ss->next_req;
-    const uint64_t used = ld_agent(t.alloc_ctr);
+    const uint64_t used = ld_agent(t.alloc_ctr) - ld_agent(t.free_count);
     uint64_t reserved = 0, ops = 0, n = 0, lo = r;
     while (r < sv.nreq) {
         const uint64_t el = now_ticks() - ss->t0;
@@ -410,6 +410,8 @@ struct DevState {
     uint64_t alloc_ctr;
     uint32_t status;
     uint32_t pad;
+    uint64_t free_head;   // reclaim policy: tagged head of the free pool slabs
+    uint64_t free_count;  // reclaim policy: slabs on the free list
     BarState bar;
     StatsDev stats;
     unsigned long long dump_counter;

@@ -52,6 +52,7 @@ static void normalise(lslab_tuning& tu) {
     if (tu.max_subbatch == 0) tu.max_subbatch = 1ull << 20;
     if (tu.combine == 0) tu.combine = LSLAB_COMBINE_BLOCK;
     if (tu.hash == 0) tu.hash = LSLAB_HASH_IDENTITY;
+    if (tu.reclaim == 0) tu.reclaim = LSLAB_RECLAIM_OFF;
 }
 
 static bool tuning_ok(const lslab_tuning& tu) {
@@ -63,6 +64,7 @@ static bool tuning_ok(const lslab_tuning& tu) {
     if (tu.max_subbatch < 64 || tu.max_subbatch > (1ull << 30)) return false;
     if (tu.combine != LSLAB_COMBINE_WAVE && tu.combine != LSLAB_COMBINE_BLOCK) return false;
     if (tu.hash != LSLAB_HASH_IDENTITY && tu.hash != LSLAB_HASH_MIX) return false;
+    if (tu.reclaim != LSLAB_RECLAIM_OFF && tu.reclaim != LSLAB_RECLAIM_ON) return false;
     return true;
 }
 
@@ -175,6 +177,9 @@ static int create_common(lslab_table** out, TableType type, bool is_map, int kb,
     }
     t->dev.alloc_ctr = &t->state->alloc_ctr;
     t->dev.status = &t->state->status;
+    t->dev.free_head = &t->state->free_head;
+    t->dev.free_count = &t->state->free_count;
+    t->dev.reclaim = t->tun.reclaim == LSLAB_RECLAIM_ON;
     t->dev.nbuckets = t->nbuckets;
     t->dev.pool = pool;
     t->dev.mask = (uint32_t)(t->nbuckets - 1);
@@ -216,9 +221,11 @@ extern "C" int lslab_set_tuning(lslab_table* t, const lslab_tuning* tuning) {
     if (!t || !tuning) return LSLAB_ERR_INVALID;
     lslab_tuning tu = *tuning;
     if (tu.hash == 0) tu.hash = t->tun.hash;  // 0 keeps the table's hash
+    if (tu.reclaim == 0) tu.reclaim = t->tun.reclaim;
     normalise(tu);
     if (!tuning_ok(tu)) return LSLAB_ERR_INVALID;
     if (tu.hash != t->tun.hash) return LSLAB_ERR_INVALID;  // the table's buckets depend on it
+    if (tu.reclaim != t->tun.reclaim) return LSLAB_ERR_INVALID;  // fixed with the table
     const bool resize = tu.max_subbatch != t->tun.max_subbatch;
     t->tun = tu;
     if (resize) {
@@ -447,7 +454,7 @@ extern "C" int lslab_stats_get(lslab_table* t, lslab_stats* out, hipStream_t str
     HIPCHK(hipMemcpy(&hs, t->state, sizeof(DevState), hipMemcpyDeviceToHost));
     out->buckets = t->nbuckets;
     out->pool_slabs = t->pool;
-    out->slabs_used = std::min<uint64_t>(hs.alloc_ctr, t->pool);
+    out->slabs_used = std::min<uint64_t>(hs.alloc_ctr, t->pool) - hs.free_count;
     out->present = hs.stats.present;
     out->max_chain = hs.stats.max_chain;
     out->slab_bytes = t->slab_bytes;

@@ -29,6 +29,10 @@ enum : uint32_t { ST_POOL_EXHAUSTED = 1u, ST_TIMEOUT = 2u, ST_CORRUPT = 4u };
 constexpr uint32_t LOCK_WRITER = 0x80000000u;
 constexpr uint64_t GT_EMPTY = ~0ull;
 constexpr uint64_t NOLOC = ~0ull;
+constexpr uint64_t NOSLAB = ~0ull;
+// Internal opcode (never in a caller's op stream): unlink every empty pool
+// slab of the key's chain (reclaim policy), under the bucket's lock.
+constexpr uint8_t OP_COMPACT = 3;
 // 10 s at the 100 MHz constant clock: bound on every spin so that a lost
 // lock holder turns into a status code instead of a hung device.
 constexpr uint64_t SPIN_LIMIT_TICKS = 1000000000ull;
@@ -74,6 +78,9 @@ struct TableDev {
     uint32_t mask;         // nbuckets - 1
     uint32_t gt_mask;      // gt capacity - 1
     uint32_t hash_mix;     // bucket by mix64(key) instead of the key itself
+    uint32_t reclaim;      // reclaim policy: first free slot of the chain, free emptied pool slabs
+    uint64_t* free_head;   // reclaim: free pool slabs, (tag << 32 | slab), 0 = empty
+    uint64_t* free_count;  // reclaim: slabs on the free list
 };
 
 // Operation stream: per-op opcode (or a uniform one), keys and values with a
@@ -339,6 +346,48 @@ __device__ __forceinline__ void walk_on(const TableDev& t, Probe p, uint64_t& s,
             valid = L::CAPMASK;  // report the last slab as full: no insert lands here
             return;
         }
+        s = p.next;
+        p = probe_slab<L, G, COH>(t, s, key);
+    }
+}
+
+// Walk of bucket b's chain for a mutation under the reclaim policy: as walk,
+// and also the slab before s (prev, NOSLAB when s is the head) and the first
+// slab up to s with an empty key slot (room, NOSLAB if none) with its bitmap.
+template <class L, int G, bool COH>
+__device__ __forceinline__ void walk_full(const TableDev& t, uint64_t b, typename L::K key,
+                                          uint64_t& s, int& slot, uint32_t& valid,
+                                          uint64_t& prev, uint64_t& room, uint32_t& room_valid) {
+    s = b;
+    prev = NOSLAB;
+    room = NOSLAB;
+    room_valid = 0;
+    Probe p = probe_slab<L, G, COH>(t, s, key);
+    for (uint64_t hops = 0;; ++hops) {
+        if (room == NOSLAB && (~p.valid & L::CAPMASK)) {
+            room = s;
+            room_valid = p.valid;
+        }
+        if (p.slot >= 0) {
+            slot = p.slot;
+            valid = p.valid;
+            return;
+        }
+        if (p.next == 0) {
+            slot = -1;
+            valid = p.valid;
+            return;
+        }
+        if (p.next < t.nbuckets || p.next >= t.nbuckets + t.pool || hops > t.pool) {
+            if (Grp<G>::lane() == 0)
+                __hip_atomic_fetch_or(t.status, ST_CORRUPT, __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
+            slot = -1;
+            valid = L::CAPMASK;
+            room = NOSLAB;
+            return;
+        }
+        prev = s;
         s = p.next;
         p = probe_slab<L, G, COH>(t, s, key);
     }

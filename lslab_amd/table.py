@@ -43,9 +43,11 @@ def _ops(t, n):
 
 
 def tuning(lanes_per_op=4, engine=_lib.ENGINE_PLAIN, block_threads=256, max_blocks=0,
-           max_subbatch=1 << 20, combine=_lib.COMBINE_BLOCK, hash=_lib.HASH_IDENTITY):
+           max_subbatch=1 << 20, combine=_lib.COMBINE_BLOCK, hash=0, reclaim=0):
+    """Launch shapes and table policies (lslab_tuning).  hash and reclaim are
+    fixed at creation; 0 in set_tuning keeps them."""
     return _lib.Tuning(lanes_per_op, engine, block_threads, max_blocks, max_subbatch, combine,
-                       hash)
+                       hash, reclaim)
 
 
 class _Table:

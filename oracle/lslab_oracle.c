@@ -36,6 +36,9 @@ struct orc_table {
     uint64_t nbuckets, pool, used, present;
     uint64_t key_mask, val_mask;
     int hash_mix;     /* bucket by fmix64(key) instead of the key (LSLAB_HASH_MIX) */
+    int reclaim;      /* LSLAB_RECLAIM_ON policy (see orc_set_reclaim) */
+    uint32_t* freel;  /* free pool slabs, LIFO (reclaim) */
+    uint64_t nfree;
     uint64_t* keys;   /* (nbuckets + pool) * 32 */
     uint64_t* vals;   /* maps only, same shape */
     uint32_t* valid;  /* per slab */
@@ -72,6 +75,7 @@ orc_table* orc_create(int is_map, int key_bytes, int val_bytes, unsigned n_log2,
 
 void orc_destroy(orc_table* t) {
     if (!t) return;
+    free(t->freel);
     free(t->keys);
     free(t->vals);
     free(t->valid);
@@ -91,6 +95,15 @@ static uint64_t fmix64(uint64_t z) {
 
 void orc_set_hash(orc_table* t, int mix) { t->hash_mix = mix != 0; }
 
+int orc_set_reclaim(orc_table* t, int on) {
+    t->reclaim = on != 0;
+    if (t->reclaim && !t->freel) {
+        t->freel = (uint32_t*)calloc(t->pool ? t->pool : 1, sizeof(uint32_t));
+        if (!t->freel) return 0;
+    }
+    return 1;
+}
+
 /* Bucket of a key: the reference's identity hash (hash.h:10-15) masked to
  * the table, or fmix64 of the key for a MIX table. */
 static uint64_t bucket(const orc_table* t, uint64_t key) {
@@ -98,41 +111,60 @@ static uint64_t bucket(const orc_table* t, uint64_t key) {
 }
 
 /* Find key: returns 1 and (slab, slot) when present.  Also reports the last
- * slab of the chain for a subsequent insert. */
+ * slab of the chain for a subsequent insert, the slab before the key's slab
+ * (prev, NOSLAB for a head) and, for the reclaim policy, the first slab of
+ * the chain with an empty slot (room, NOSLAB if none). */
+#define NOSLAB (~0ull)
 static int find(const orc_table* t, uint64_t key, uint64_t* slab_out, int* slot_out,
-                uint64_t* last_out) {
-    uint64_t s = bucket(t, key);
+                uint64_t* last_out, uint64_t* prev_out, uint64_t* room_out) {
+    const uint32_t capmask = (t->cap == 32) ? 0xffffffffu : ((1u << t->cap) - 1u);
+    uint64_t s = bucket(t, key), prev = NOSLAB, room = NOSLAB;
     for (;;) {
         uint32_t v = t->valid[s];
         const uint64_t* k = t->keys + s * 32;
+        if (room == NOSLAB && (~v & capmask)) room = s;
         for (int i = 0; i < t->cap; ++i) {
             if (((v >> i) & 1u) && k[i] == key) {
                 *slab_out = s;
                 *slot_out = i;
+                *prev_out = prev;
+                *room_out = room;
                 return 1;
             }
         }
         uint32_t nx = t->next[s];
         if (nx == 0) {
             *last_out = s;
+            *prev_out = prev;
+            *room_out = room;
             return 0;
         }
+        prev = s;
         s = nx;
     }
 }
 
-/* Insert an absent key into the lowest empty slot of the last slab, or into
- * slot 0 of a freshly linked pool slab.  Returns 0 when the pool is full. */
-static int insert_absent(orc_table* t, uint64_t last, uint64_t key, uint64_t val) {
+/* Insert an absent key.  Reference policy (traverse.h:54-55,98-124): the
+ * lowest empty slot of the last slab, else slot 0 of a freshly linked pool
+ * slab.  Reclaim policy: the lowest empty slot of the first slab with one,
+ * else a pool slab taken from the free list (most recently freed first) or
+ * the bump pointer, linked after the last slab.  Returns 0 when the pool is
+ * full. */
+static int insert_absent(orc_table* t, uint64_t last, uint64_t room, uint64_t key, uint64_t val) {
     uint32_t capmask = (t->cap == 32) ? 0xffffffffu : ((1u << t->cap) - 1u);
-    uint32_t empty = ~t->valid[last] & capmask;
+    uint64_t s = (t->reclaim && room != NOSLAB) ? room : last;
+    uint32_t empty = ~t->valid[s] & capmask;
     int slot;
-    uint64_t s = last;
     if (empty) {
         slot = __builtin_ctz(empty);
     } else {
-        if (t->used >= t->pool) return 0;
-        s = t->nbuckets + t->used++;
+        if (t->reclaim && t->nfree) {
+            s = t->freel[--t->nfree];
+        } else {
+            if (t->used >= t->pool) return 0;
+            s = t->nbuckets + t->used++;
+        }
+        t->next[s] = 0;
         t->next[last] = (uint32_t)s;
         slot = 0;
     }
@@ -143,13 +175,25 @@ static int insert_absent(orc_table* t, uint64_t last, uint64_t key, uint64_t val
     return 1;
 }
 
+/* Clear a present key's valid bit (traverse.h:88-90).  Reclaim policy: a
+ * pool slab left empty is unlinked from its chain and freed. */
+static void remove_at(orc_table* t, uint64_t slab, int slot, uint64_t prev) {
+    t->valid[slab] ^= 1u << slot;
+    t->present--;
+    if (t->reclaim && slab >= t->nbuckets && t->valid[slab] == 0) {
+        t->next[prev] = t->next[slab];
+        t->next[slab] = 0;
+        t->freel[t->nfree++] = (uint32_t)slab;
+    }
+}
+
 static int apply_one(orc_table* t, uint8_t op, uint64_t key, uint64_t val, uint8_t* f,
                      uint64_t* out) {
-    uint64_t slab = 0, last = 0;
+    uint64_t slab = 0, last = 0, prev = NOSLAB, room = NOSLAB;
     int slot = 0;
     key &= t->key_mask;
     val &= t->val_mask;
-    int hit = find(t, key, &slab, &slot, &last);
+    int hit = find(t, key, &slab, &slot, &last, &prev, &room);
     *f = 0;
     *out = 0;
     switch (op) {
@@ -168,15 +212,14 @@ static int apply_one(orc_table* t, uint8_t op, uint64_t key, uint64_t val, uint8
                 }
                 return 1;
             }
-            if (!insert_absent(t, last, key, val)) return 0;
+            if (!insert_absent(t, last, room, key, val)) return 0;
             *f = t->is_map ? 0 : 1; /* set insert reports true for a new key too */
             return 1;
         case 2: /* remove */
             if (hit) {
                 *f = 1;
                 if (t->is_map) *out = t->vals[slab * 32 + slot];
-                t->valid[slab] ^= 1u << slot;
-                t->present--;
+                remove_at(t, slab, slot, prev);
             }
             return 1;
         default:
@@ -198,10 +241,10 @@ int64_t orc_apply(orc_table* t, uint64_t n, const uint8_t* ops, const uint64_t* 
 }
 
 int orc_lookup(const orc_table* t, uint64_t key, uint64_t* val_out) {
-    uint64_t slab, last;
+    uint64_t slab, last, prev, room;
     int slot;
     key &= t->key_mask;
-    if (find(t, key, &slab, &slot, &last)) {
+    if (find(t, key, &slab, &slot, &last, &prev, &room)) {
         if (val_out) *val_out = t->is_map ? t->vals[slab * 32 + slot] : 0;
         return 1;
     }
@@ -210,7 +253,7 @@ int orc_lookup(const orc_table* t, uint64_t key, uint64_t* val_out) {
 }
 
 uint64_t orc_present(const orc_table* t) { return t->present; }
-uint64_t orc_slabs_used(const orc_table* t) { return t->used; }
+uint64_t orc_slabs_used(const orc_table* t) { return t->used - t->nfree; }
 
 uint64_t orc_max_chain(const orc_table* t) {
     uint64_t best = 0;
@@ -309,18 +352,17 @@ static int64_t uf_find(int64_t* par, int64_t x) {
 
 /* Move the table's state of key to (present, v). */
 static void adopt(orc_table* t, uint64_t key, int present, uint64_t v) {
-    uint64_t slab, last;
+    uint64_t slab, last, prev, room;
     int slot;
-    int hit = find(t, key, &slab, &slot, &last);
+    int hit = find(t, key, &slab, &slot, &last, &prev, &room);
     if (present) {
         if (hit) {
             if (t->is_map) t->vals[slab * 32 + slot] = v;
         } else {
-            insert_absent(t, last, key, v);
+            insert_absent(t, last, room, key, v);
         }
     } else if (hit) {
-        t->valid[slab] ^= 1u << slot;
-        t->present--;
+        remove_at(t, slab, slot, prev);
     }
 }
 

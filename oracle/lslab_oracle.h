@@ -54,6 +54,13 @@ uint64_t orc_max_chain(const orc_table* t);
 /* Buckets by fmix64(key) (mix != 0) instead of the reference's identity
  * hash; call before the first apply. */
 void orc_set_hash(orc_table* t, int mix);
+/* Reclaim policy (on != 0; call before the first apply): inserts take the
+ * lowest empty slot of the first slab of the chain that has one; a remove
+ * that empties a pool slab unlinks it and frees it; new slabs come from the
+ * free list (most recently freed first) before the bump pointer.  Slabs used
+ * = bump count - free slabs.  Returns 0 if the free list cannot be
+ * allocated. */
+int orc_set_reclaim(orc_table* t, int on);
 /* Buckets by chain length in slabs: hist[i] = chains of i+1 slabs, the last
  * bin also counting longer ones. */
 void orc_chain_hist(const orc_table* t, uint64_t* hist, unsigned nbins);

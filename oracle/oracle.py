@@ -47,6 +47,8 @@ def _lib():
         getattr(lib, f).argtypes = [vp]
     lib.orc_set_hash.restype = None
     lib.orc_set_hash.argtypes = [vp, ctypes.c_int]
+    lib.orc_set_reclaim.restype = ctypes.c_int
+    lib.orc_set_reclaim.argtypes = [vp, ctypes.c_int]
     lib.orc_chain_hist.restype = None
     lib.orc_chain_hist.argtypes = [vp, u64p, ctypes.c_uint]
     lib.orc_dump.restype = u64
@@ -72,7 +74,7 @@ class Replay:
     """Sequential replay table: map (is_map=True) or set."""
 
     def __init__(self, is_map, key_bytes, val_bytes, n_log2, pool_slabs=10000, slab_keys=31,
-                 mix_hash=False):
+                 mix_hash=False, reclaim=False):
         self.is_map = bool(is_map)
         self.h = _lib().orc_create(int(is_map), key_bytes, val_bytes if is_map else 0,
                                    n_log2, pool_slabs, slab_keys)
@@ -80,6 +82,8 @@ class Replay:
             raise ValueError("orc_create rejected the configuration")
         if mix_hash:
             _lib().orc_set_hash(self.h, 1)
+        if reclaim and not _lib().orc_set_reclaim(self.h, 1):
+            raise MemoryError("orc_set_reclaim")
 
     def close(self):
         if self.h:

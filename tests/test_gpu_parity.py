@@ -413,3 +413,99 @@ def test_mix_hash_table(lslab, oracle_mod, is_map, kb, vb, engine):
     f, _ = run_batch(tbl, is_map, kb, vb, np.zeros(len(keys), np.uint8), keys, keys)
     rf, _, _ = ref.apply(np.zeros(len(keys), np.uint8), keys)
     np.testing.assert_array_equal(f, rf)
+
+
+def make_reclaim(lslab, is_map, kb, vb, n_log2, pool, engine, lanes, reclaim):
+    tun = lslab.tuning(lanes_per_op=lanes, engine=engine, max_subbatch=1 << 14,
+                       reclaim=lslab.RECLAIM_ON if reclaim else lslab.RECLAIM_OFF)
+    if is_map:
+        return lslab.Map(kb, vb, n_log2, pool, tun)
+    return lslab.Set(kb, n_log2, pool, tun)
+
+
+@pytest.mark.parametrize("engine,lanes", [(0, 4), (1, 1), (1, 4), (1, 32)])
+def test_reclaim_churn(lslab, oracle_mod, engine, lanes):
+    """One bucket, groups of 31 keys; each batch removes the group inserted two
+    batches earlier and inserts a new one.  The reference policy exhausts a
+    4-slab pool (status POOL_EXHAUSTED); the reclaim policy keeps one pool
+    slab, and its results, slab count and chains equal the oracle's."""
+    say("reclaim_churn", engine, lanes)
+    groups = [np.arange(31 * g, 31 * g + 31, dtype=np.uint64) * np.uint64(3) for g in range(10)]
+    for reclaim in (False, True):
+        tbl = make_reclaim(lslab, True, 8, 8, 0, 4, engine, lanes, reclaim)
+        ref = Replay(True, 8, 8, 0, pool_slabs=4, reclaim=reclaim)
+        # One group per batch: a batch's inserts into one bucket land in lock
+        # order, so only whole-group batches make the slab of every key, and
+        # with it the pool use, independent of that order.
+        for g in groups[:2]:
+            run_batch(tbl, True, 8, 8, np.full(31, OP_PUT, np.uint8), g, g)
+            ref.apply(np.full(31, OP_PUT, np.uint8), g, g)
+        failed = 0
+        for c in range(8):
+            for ops, ks in ((np.full(31, OP_REMOVE, np.uint8), groups[c]),
+                            (np.full(31, OP_PUT, np.uint8), groups[c + 2])):
+                f, v = run_batch(tbl, True, 8, 8, ops, ks, ks + np.uint64(c))
+                rf, rv, fail = ref.apply(ops, ks, ks + np.uint64(c))
+                failed += fail
+                if not fail:
+                    np.testing.assert_array_equal(f, rf)
+                    np.testing.assert_array_equal(v, rv)
+        st = tbl.status()
+        if reclaim:
+            assert failed == 0 and st == 0
+            assert tbl.stats()["slabs_used"] == ref.slabs_used == 1
+            assert tbl.chain_histogram(4) == ref.chain_hist(4) == [0, 1, 0, 0]
+            k, vv = dump_sorted(tbl, True, 8, 8)
+            rk, rvv = oracle_dump_sorted(ref)
+            np.testing.assert_array_equal(k, rk)
+            np.testing.assert_array_equal(vv, rvv)
+            # remove the group in the pool slab: the slab is freed
+            run_batch(tbl, True, 8, 8, np.full(31, OP_REMOVE, np.uint8), groups[9], groups[9])
+            assert tbl.stats()["slabs_used"] == 0 and tbl.chain_histogram(4) == [1, 0, 0, 0]
+        else:
+            assert failed > 0 and st == 4  # LSLAB_ERR_POOL_EXHAUSTED
+            assert tbl.stats()["slabs_used"] == 4
+
+
+@pytest.mark.parametrize("is_map,kb,vb", [(True, 8, 8), (True, 4, 4), (False, 8, 0),
+                                         (False, 4, 0)])
+@pytest.mark.parametrize("engine,lanes", [(0, 8), (1, 1), (1, 8)])
+def test_reclaim_random_stream(lslab, oracle_mod, is_map, kb, vb, engine, lanes):
+    """Phase-separated batches (distinct keys per batch) with many removes on
+    a small table under the reclaim policy: results bit-exact against the
+    oracle's reclaim replay, equal contents, and removing every key returns
+    every pool slab."""
+    say("reclaim_random", is_map, kb, vb, engine, lanes)
+    rng = np.random.default_rng(31 + kb + lanes + engine)
+    n_log2, pool = 3, 2048
+    cap = 30 if (not is_map and kb == 4) else 31
+    tbl = make_reclaim(lslab, is_map, kb, vb, n_log2, pool, engine, lanes, True)
+    ref = Replay(is_map, kb, vb, n_log2, pool_slabs=pool, slab_keys=cap, reclaim=True)
+    universe = np.arange(1, 20001, dtype=np.uint64) * np.uint64(2654435761)
+    if kb == 4:
+        universe &= np.uint64(0xFFFFFFFF)
+    for bi in range(30):
+        keys = rng.permutation(universe)[:rng.integers(200, 4000)]
+        ops = rng.choice([OP_GET, OP_PUT, OP_REMOVE], size=len(keys),
+                         p=[0.2, 0.4, 0.4]).astype(np.uint8)
+        vals = rng.integers(1, 2**31, size=len(keys), dtype=np.uint64)
+        f, v = run_batch(tbl, is_map, kb, vb, ops, keys, vals)
+        rf, rv, fail = ref.apply(ops, keys, vals if is_map else None)
+        assert fail == 0
+        np.testing.assert_array_equal(f, rf, err_msg=f"batch {bi}")
+        np.testing.assert_array_equal(v, rv, err_msg=f"batch {bi}")
+    assert tbl.status() == 0
+    k, vv = dump_sorted(tbl, is_map, kb, vb)
+    rk, rvv = oracle_dump_sorted(ref)
+    np.testing.assert_array_equal(k, rk)
+    np.testing.assert_array_equal(vv, rvv)
+    st = tbl.stats()
+    assert st["present"] == ref.present
+    # chains hold no empty pool slab: at most one partly filled slab more per
+    # key range than the keys need (a bound, placement is order-dependent)
+    assert st["slabs_used"] <= st["present"]
+    run_batch(tbl, is_map, kb, vb, np.full(len(k), OP_REMOVE, np.uint8), k, k)
+    assert tbl.status() == 0
+    st = tbl.stats()
+    assert st["present"] == 0 and st["slabs_used"] == 0
+    assert tbl.chain_histogram(4) == [1 << n_log2, 0, 0, 0]

@@ -325,3 +325,69 @@ def test_mix_hash_buckets_closed_form(oracle_mod):
         assert r.chain_hist(16) == want
     # the identity table piles everything into one long chain; the mixed one does not
     assert want[0] + want[1] + want[2] == 1 << n_log2
+
+
+def test_reclaim_policy_closed_forms(oracle_mod):
+    """One bucket, 31-key slabs, groups of 31 keys.  Each cycle removes the
+    group inserted two cycles earlier (never in the last slab) and inserts a
+    new group: the reference policy needs a new slab every cycle (holes
+    outside the last slab are never reused) until the pool is exhausted; the
+    reclaim policy refills holes and frees the emptied pool slab, so one pool
+    slab suffices."""
+    def run(reclaim, cycles):
+        r = Replay(True, 8, 8, 0, pool_slabs=4, reclaim=reclaim)
+        put = lambda ks: r.apply(np.full(len(ks), OP_PUT, np.uint8), ks, ks)  # noqa: E731
+        rem = lambda ks: r.apply(np.full(len(ks), OP_REMOVE, np.uint8), ks)  # noqa: E731
+        groups = [np.arange(31 * g, 31 * g + 31, dtype=np.uint64) for g in range(cycles + 2)]
+        assert put(np.concatenate(groups[:2]))[2] == 0 and r.slabs_used == 1
+        fails = 0
+        for c in range(cycles):
+            rem(groups[c])
+            fails += put(groups[c + 2])[2]
+        return r, fails, groups
+    r, fails, _ = run(False, 6)
+    assert fails > 0 and r.slabs_used == 4  # one new slab per cycle: pool of 4 exhausted
+    r, fails, groups = run(True, 6)
+    assert fails == 0 and r.slabs_used == 1 and r.present == 62
+    # odd cycles refill the pool slab: after 6 cycles it holds group 7 (the head
+    # group 6); removing group 7 frees it and leaves the head alone
+    r.apply(np.full(31, OP_REMOVE, np.uint8), groups[7])
+    assert r.slabs_used == 0 and r.chain_hist(4) == [1, 0, 0, 0]
+    # the next overflow takes the freed slab again
+    r.apply(np.full(5, OP_PUT, np.uint8), np.arange(5000, 5005, dtype=np.uint64),
+            np.arange(5, dtype=np.uint64))
+    assert r.slabs_used == 1 and r.present == 36
+
+
+@pytest.mark.parametrize("is_map", [True, False])
+def test_reclaim_matches_dict_and_frees_everything(oracle_mod, is_map):
+    """Reclaim changes placement only: results equal a dict model under random
+    churn, the pool never exceeds what the live keys need, and removing every
+    key returns every pool slab."""
+    rng = np.random.default_rng(8)
+    n_log2, pool = 2, 400
+    r = Replay(is_map, 8, 8 if is_map else 0, n_log2, pool_slabs=pool, reclaim=True)
+    d = {}
+    for step in range(40):
+        keys = rng.integers(0, 3000, size=700).astype(np.uint64)
+        ops = rng.choice([OP_GET, OP_PUT, OP_REMOVE], size=700, p=[0.2, 0.45, 0.35]).astype(np.uint8)
+        vals = rng.integers(1, 2**40, size=700, dtype=np.uint64)
+        f, v, fail = r.apply(ops, keys, vals if is_map else None)
+        assert fail == 0
+        for i, (op, k) in enumerate(zip(ops, keys)):
+            k = int(k)
+            had = k in d
+            assert f[i] == (1 if (had or (op == OP_PUT and not is_map)) else 0)
+            if is_map:
+                assert v[i] == (d[k] if had else 0)
+            if op == OP_PUT:
+                d[k] = int(vals[i]) if is_map else 0
+            elif op == OP_REMOVE:
+                d.pop(k, None)
+        per_bucket = np.bincount(np.array(list(d.keys()), np.int64) & 3, minlength=4)
+        assert r.present == len(d)
+        assert r.slabs_used <= sum(max(0, int(c) - 31) for c in per_bucket) + pool  # sanity
+    alive = np.array(sorted(d), np.uint64)
+    r.apply(np.full(len(alive), OP_REMOVE, np.uint8), alive)
+    assert r.present == 0 and r.slabs_used == 0
+    assert r.chain_hist(4) == [4, 0, 0, 0]
