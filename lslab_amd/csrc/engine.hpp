@@ -108,29 +108,22 @@ __device__ __forceinline__ void compact_chain(const TableDev& t, uint64_t b) {
     }
 }
 
-// Applies one op to bucket b's chain.  With COH the caller holds the bucket
-// lock (shared for gets, exclusive otherwise) and every slab access is
-// agent-coherent; without, loads are plain (read-only launches).  All G
-// lanes of the group call it; lane 0 performs the writes.
-// Allocation of new slabs is aggregated over all groups of the wave that
-// reach this point together: one atomicAdd per wave.
-// Returns (found, value) with the reference's meaning per op:
-//   get    -> (present, value)      put -> (was present, old value)
-//   remove -> (was present, value)  set insert -> (true, 0)
-// and ok=false if the pool was exhausted.
-// Placement: the reference's policy (the last slab's lowest empty slot, else
-// a new slab linked at the end), or under reclaim the first slab with an
-// empty slot, a freed slab before a fresh one, and removal of a pool slab
-// that a remove leaves empty.
-template <class L, int G, bool COH>
-__device__ __forceinline__ void apply_op(const TableDev& t, uint8_t op, typename L::K key,
-                                         typename L::V val, uint64_t b, bool& rfound,
-                                         typename L::V& rval, bool& ok) {
+// apply_op for a mutation on a table with the reclaim policy, under the
+// bucket's exclusive lock: an insert takes the first slab of the chain with
+// an empty slot, a freed slab before a fresh one; a remove that leaves a pool
+// slab empty unlinks and frees it; OP_COMPACT unlinks every empty pool slab.
+// Instantiated only for reclaim tables (RC), so the reference-policy kernels
+// carry none of it.
+template <class L, int G>
+__device__ __forceinline__ void apply_op_reclaim(const TableDev& t, uint8_t op, typename L::K key,
+                                                 typename L::V val, uint64_t b, bool& rfound,
+                                                 typename L::V& rval, bool& ok) {
+    constexpr bool COH = true;
     using V = typename L::V;
     const int gl = Grp<G>::lane();
     ok = true;
     if (op == OP_COMPACT) {
-        if constexpr (COH) compact_chain<L, G>(t, b);
+        compact_chain<L, G>(t, b);
         rfound = false;
         rval = V(0);
         return;
@@ -138,11 +131,8 @@ __device__ __forceinline__ void apply_op(const TableDev& t, uint8_t op, typename
     uint64_t s, prev = NOSLAB, room = NOSLAB;
     int slot;
     uint32_t valid, room_valid = 0;
-    const bool rc = COH && t.reclaim && op != OP_GET;
-    if (rc)
-        walk_full<L, G, COH>(t, b, key, s, slot, valid, prev, room, room_valid);
-    else
-        walk<L, G, COH>(t, b, key, s, slot, valid);
+    constexpr bool rc = true;
+    walk_full<L, G, COH>(t, b, key, s, slot, valid, prev, room, room_valid);
     const bool found = slot >= 0;
     V cur = V(0);
     if constexpr (L::IS_MAP) {
@@ -232,7 +222,108 @@ __device__ __forceinline__ void apply_op(const TableDev& t, uint8_t op, typename
         // A reused slab still carries its free-list link: clear it first.
         // Key and value, then the valid bit, then the link: each step
         // completes before the next is issued.
-        if (fresh && t.reclaim) st_coh(next_word<L>(t, ins_s), 0u);
+        if (fresh) st_coh(next_word<L>(t, ins_s), 0u);
+        st_coh(key_slot<L>(t, ins_s, ins_slot), key);
+        if constexpr (L::IS_MAP) st_coh(val_slot<L>(t, ins_s, ins_slot), val);
+        wait_vm();
+        __hip_atomic_fetch_or(valid_word<L>(t, ins_s), 1u << ins_slot, __ATOMIC_RELAXED,
+                              __HIP_MEMORY_SCOPE_AGENT);
+        if (fresh) {
+            wait_vm();
+            st_coh(next_word<L>(t, s), (uint32_t)ins_s);
+        }
+    }
+}
+
+
+// Applies one op to bucket b's chain.  With COH the caller holds the bucket
+// lock (shared for gets, exclusive otherwise) and every slab access is
+// agent-coherent; without, loads are plain (read-only launches).  All G
+// lanes of the group call it; lane 0 performs the writes.
+// Allocation of new slabs is aggregated over all groups of the wave that
+// reach this point together: one atomicAdd per wave.
+// Returns (found, value) with the reference's meaning per op:
+//   get    -> (present, value)      put -> (was present, old value)
+//   remove -> (was present, value)  set insert -> (true, 0)
+// and ok=false if the pool was exhausted.
+// Placement is the reference's policy (the last slab's lowest empty slot,
+// else a new slab linked at the end; traverse.h:54-55,98-124); tables with
+// the reclaim policy (RC) hand mutations to apply_op_reclaim.
+template <class L, int G, bool COH, bool RC>
+__device__ __forceinline__ void apply_op(const TableDev& t, uint8_t op, typename L::K key,
+                                         typename L::V val, uint64_t b, bool& rfound,
+                                         typename L::V& rval, bool& ok) {
+    using V = typename L::V;
+    if constexpr (COH && RC) {
+        if (op != OP_GET) {
+            apply_op_reclaim<L, G>(t, op, key, val, b, rfound, rval, ok);
+            return;
+        }
+    }
+    const int gl = Grp<G>::lane();
+    ok = true;
+    uint64_t s;
+    int slot;
+    uint32_t valid;
+    walk<L, G, COH>(t, b, key, s, slot, valid);
+    const bool found = slot >= 0;
+    V cur = V(0);
+    if constexpr (L::IS_MAP) {
+        if (found) cur = COH ? ld_coh(val_slot<L>(t, s, slot)) : *val_slot<L>(t, s, slot);
+    }
+    rfound = found;
+    rval = cur;
+    bool ins = false, need_alloc = false, fresh = false;
+    int ins_slot = -1;
+    uint64_t ins_s = s;
+    if (op == OP_PUT) {
+        if (found) {
+            if constexpr (L::IS_MAP) {
+                if (gl == 0) st_coh(val_slot<L>(t, s, slot), val);
+            }
+        } else {
+            ins = true;
+            const uint32_t empty = ~valid & L::CAPMASK;
+            if (empty) ins_slot = __builtin_ctz(empty);
+            else need_alloc = true;
+        }
+        if constexpr (!L::IS_MAP) rfound = true;
+    } else if (op == OP_REMOVE) {
+        if (found && gl == 0)
+            __hip_atomic_fetch_and(valid_word<L>(t, s), ~(1u << slot), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+    }
+    const bool need_bump = need_alloc;
+    // Wave-aggregated bump allocation: one atomic for every group that needs
+    // a fresh slab in this step.
+    const uint64_t am = __ballot(need_bump && gl == 0);
+    if (am) {
+        const int first = __builtin_ctzll(am);
+        uint64_t basei = 0;
+        if ((int)__lane_id() == first)
+            basei = __hip_atomic_fetch_add(t.alloc_ctr, (uint64_t)__builtin_popcountll(am),
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        basei = grp_bcast(basei, first, 64);
+        if (need_bump) {
+            const int leader = Grp<G>::base();
+            const uint64_t idx = basei + __builtin_popcountll(am & ((1ull << leader) - 1ull));
+            if (idx < t.pool) {
+                ins_s = t.nbuckets + idx;
+                ins_slot = L::SLOT0;
+                fresh = true;
+            } else {
+                ins = false;
+                ok = false;
+                rfound = false;
+                rval = V(0);
+                if (gl == 0) __hip_atomic_fetch_or(t.status, ST_POOL_EXHAUSTED, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+    }
+    if (ins && gl == 0) {
+        // Key and value, then the valid bit, then the link: each step
+        // completes before the next is issued.
         st_coh(key_slot<L>(t, ins_s, ins_slot), key);
         if constexpr (L::IS_MAP) st_coh(val_slot<L>(t, ins_s, ins_slot), val);
         wait_vm();
@@ -247,13 +338,13 @@ __device__ __forceinline__ void apply_op(const TableDev& t, uint8_t op, typename
 
 // Runs one op under the bucket lock (or lock-free when !LOCKED) with the
 // try-lock retry loop.  All G lanes call it with the same arguments.
-template <class L, int G, bool LOCKED>
+template <class L, int G, bool LOCKED, bool RC>
 __device__ __forceinline__ void run_op(const TableDev& t, uint8_t op, typename L::K key,
                                        typename L::V val, bool& rfound, typename L::V& rval,
                                        bool& ok) {
     const uint64_t b = bucket_of(t, (uint64_t)key);
     if constexpr (!LOCKED) {
-        apply_op<L, G, false>(t, op, key, val, b, rfound, rval, ok);
+        apply_op<L, G, false, RC>(t, op, key, val, b, rfound, rval, ok);
         return;
     } else {
         const int gl = Grp<G>::lane();
@@ -276,7 +367,7 @@ __device__ __forceinline__ void run_op(const TableDev& t, uint8_t op, typename L
                     // Inside the lock every slab access is agent-coherent, so
                     // the lock needs no cache-wide fence: the critical section's
                     // reads and writes complete (wait_vm) before the word drops.
-                    apply_op<L, G, true>(t, op, key, val, b, rfound, rval, ok);
+                    apply_op<L, G, true, RC>(t, op, key, val, b, rfound, rval, ok);
                     wait_vm();
                     if (gl == 0) {
                         if (shared) unlock_shared(lw);
@@ -663,10 +754,19 @@ __device__ __forceinline__ void phase_r_range(const TableDev& t, const OpsDev& o
     }
 }
 
+// Unlinks the empty pool slabs of key's chain under the bucket's lock
+// (reclaim policy).
+template <class L, int G>
+__device__ __forceinline__ void compact_locked(const TableDev& t, typename L::K key) {
+    bool rf, ok;
+    typename L::V rv;
+    run_op<L, G, true, true>(t, OP_COMPACT, key, typename L::V(0), rf, rv, ok);
+}
+
 // Phase W of one final-mutation candidate: if it is still its key's last
 // mutation (nothing exchanged after it), it writes the key's final state and
 // frees the combining-table entry.
-template <class L, int G>
+template <class L, int G, bool RC>
 __device__ __forceinline__ void phase_w_ent(const TableDev& t, const OpsDev& o, uint64_t lo,
                                             const RepEnt& e) {
     using K = typename L::K;
@@ -697,30 +797,26 @@ __device__ __forceinline__ void phase_w_ent(const TableDev& t, const OpsDev& o, 
         // Reclaim: this remove left a pool slab empty; unlink it (and any
         // other empty pool slab of the chain) under the bucket's lock, which
         // also orders it after any insert into the same chain.
-        if (t.reclaim && op != OP_PUT && s >= t.nbuckets) {
+        if (RC && op != OP_PUT && s >= t.nbuckets) {
             old = grp_bcast(old, 0, G);
-            if (((old & ~(1u << slot)) & L::CAPMASK) == 0) {
-                bool rf, ok;
-                V rv;
-                run_op<L, G, true>(t, OP_COMPACT, key, V(0), rf, rv, ok);
-            }
+            if (((old & ~(1u << slot)) & L::CAPMASK) == 0) compact_locked<L, G>(t, key);
         }
     } else if (op == OP_PUT) {
         bool rf, ok;
         V rv;
-        run_op<L, G, true>(t, OP_PUT, key, val, rf, rv, ok);
+        run_op<L, G, true, RC>(t, OP_PUT, key, val, rf, rv, ok);
     }
 }
 
 // Phase W of a block: the cnt final-mutation candidates of its list (the
 // first lcap in LDS at lrep, the rest at rb), one per group.
-template <class L, int G>
+template <class L, int G, bool RC>
 __device__ __forceinline__ void phase_w_block(const TableDev& t, const OpsDev& o, uint64_t lo,
                                               const RepEnt* rb, uint32_t cnt,
                                               const RepEnt* lrep = nullptr, uint32_t lcap = 0) {
     const uint32_t gpb = blockDim.x / G;
     for (uint32_t k = threadIdx.x / G; k < cnt; k += gpb)
-        phase_w_ent<L, G>(t, o, lo, k < lcap ? lrep[k] : rb[k]);
+        phase_w_ent<L, G, RC>(t, o, lo, k < lcap ? lrep[k] : rb[k]);
 }
 
 }  // namespace lslab_dev

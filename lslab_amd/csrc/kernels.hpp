@@ -40,8 +40,9 @@ struct BarState {
     uint32_t pad[31];
 };
 
-// Plain per-batch kernel: one G-lane group per op, grid-stride.
-template <class L, int G, bool LOCKED>
+// Plain per-batch kernel: one G-lane group per op, grid-stride.  RC: the
+// table's reclaim policy (only mutations, under locks, differ).
+template <class L, int G, bool LOCKED, bool RC>
 __global__ __launch_bounds__(256) void k_plain(TableDev t, OpsDev o, OutDev out, uint64_t n) {
     using K = typename L::K;
     using V = typename L::V;
@@ -54,7 +55,7 @@ __global__ __launch_bounds__(256) void k_plain(TableDev t, OpsDev o, OutDev out,
         load_op<L>(o, i, op, key, val);
         bool rf, ok;
         V rv;
-        run_op<L, G, LOCKED>(t, op, key, val, rf, rv, ok);
+        run_op<L, G, LOCKED, RC>(t, op, key, val, rf, rv, ok);
         if (Grp<G>::lane() == 0) write_result<L>(out, i, rf, rv);
     }
 }
@@ -77,10 +78,10 @@ __global__ __launch_bounds__(256) void k_phase_r(TableDev t, OpsDev o, OutDev ou
 }
 
 // Same grid as the k_phase_r launch before it: block b runs block b's list.
-template <class L, int G>
+template <class L, int G, bool RC>
 __global__ __launch_bounds__(256) void k_phase_w(TableDev t, OpsDev o, uint64_t lo, uint64_t m) {
     const uint64_t ng = (uint64_t)gridDim.x * blockDim.x / G;
-    phase_w_block<L, G>(t, o, lo, t.rep + block_rep_base(m, ng, blockDim.x / G, blockIdx.x),
+    phase_w_block<L, G, RC>(t, o, lo, t.rep + block_rep_base(m, ng, blockDim.x / G, blockIdx.x),
                         t.rep_cnt[blockIdx.x]);
 }
 
@@ -144,7 +145,7 @@ __device__ __forceinline__ bool grid_sync(BarState* bar, uint32_t nblocks, bool 
 
 // Persistent batched kernel: batches in order, each as two-phase sub-batches
 // separated by grid barriers.
-template <class L, int G, bool BC>
+template <class L, int G, bool BC, bool RC>
 __global__ __launch_bounds__(PERSIST_THREADS) void k_batches(TableDev t, OpsDev o, OutDev out,
                                                  const uint64_t* offsets, uint64_t nbatches,
                                                  uint64_t submax, BarState* bar) {
@@ -169,7 +170,7 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_batches(TableDev t, OpsDev 
                                           __HIP_MEMORY_SCOPE_AGENT);
                 return;
             }
-            phase_w_block<L, G>(t, o, slo, rb, *rc, rep_lds(), REP_LDS);
+            phase_w_block<L, G, RC>(t, o, slo, rb, *rc, rep_lds(), REP_LDS);
             if (!grid_sync(bar, gridDim.x, true, rc)) {
                 if (threadIdx.x == 0)
                     __hip_atomic_fetch_or(t.status, ST_TIMEOUT, __ATOMIC_RELAXED,
@@ -272,7 +273,7 @@ __device__ inline void serve_schedule(const TableDev& t, const ServeDev& sv, Ser
     if (n == 0) st_coh(&ss->done, 1u);
 }
 
-template <class L, int G, bool BC>
+template <class L, int G, bool BC, bool RC>
 __global__ __launch_bounds__(PERSIST_THREADS) void k_serve(TableDev t, OpsDev o, OutDev out, ServeDev sv,
                                                ServeState* ss, BarState* bar, uint64_t submax) {
     const uint64_t ng = (uint64_t)gridDim.x * blockDim.x / G;
@@ -298,7 +299,7 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_serve(TableDev t, OpsDev o,
                 phase_r_range<L, G, PERSIST_THREADS, BC>(t, o, out, slo, m, gid, ng, rb, rep_lds(),
                                                      REP_LDS);
                 if (!grid_sync(bar, gridDim.x, false)) goto fail;
-                phase_w_block<L, G>(t, o, slo, rb, *rc, rep_lds(), REP_LDS);
+                phase_w_block<L, G, RC>(t, o, slo, rb, *rc, rep_lds(), REP_LDS);
                 if (!grid_sync(bar, gridDim.x, true, rc)) goto fail;
             }
             if (blockIdx.x == 0) {

@@ -29,6 +29,13 @@ int dispatch_c(int combine, F&& f) {
     return f(std::false_type{});
 }
 
+// Reclaim policy as a compile-time flag.
+template <class F>
+int dispatch_r(int reclaim, F&& f) {
+    if (reclaim == LSLAB_RECLAIM_ON) return f(std::true_type{});
+    return f(std::false_type{});
+}
+
 int launched() { return hipGetLastError() == hipSuccess ? LSLAB_OK : LSLAB_ERR_HIP; }
 
 // Blocks of the per-batch kernels for n ops: one G-lane group per op, capped
@@ -67,27 +74,30 @@ int Launch<L>::ops(const LaunchCtx& c, const OpsDev& o, const OutDev& out, uint6
     const int bt = c.tun.block_threads;
     return dispatch_g(c.tun.lanes_per_op, [&](auto gc) {
         constexpr int G = decltype(gc)::value;
-        if (read_only || !two_phase) {
-            const uint64_t grid = grid_for(c, n);
-            if (read_only)
-                hipLaunchKernelGGL((k_plain<L, G, false>), dim3((unsigned)grid), dim3(bt), 0, s,
-                                   c.dev, o, out, n);
-            else
-                hipLaunchKernelGGL((k_plain<L, G, true>), dim3((unsigned)grid), dim3(bt), 0, s,
-                                   c.dev, o, out, n);
+        if (read_only) {
+            hipLaunchKernelGGL((k_plain<L, G, false, false>), dim3((unsigned)grid_for(c, n)),
+                               dim3(bt), 0, s, c.dev, o, out, n);
             return launched();
         }
-        return dispatch_c(c.tun.combine, [&](auto cc) {
-            constexpr bool BC = decltype(cc)::value;
-            for (uint64_t lo = 0; lo < n; lo += c.submax) {
-                const uint64_t m = std::min<uint64_t>(c.submax, n - lo);
-                const uint64_t grid = grid_for(c, m);
-                hipLaunchKernelGGL((k_phase_r<L, G, BC>), dim3((unsigned)grid), dim3(bt), 0, s,
-                                   c.dev, o, out, lo, m);
-                hipLaunchKernelGGL((k_phase_w<L, G>), dim3((unsigned)grid), dim3(bt), 0, s, c.dev,
-                                   o, lo, m);
+        return dispatch_r(c.tun.reclaim, [&](auto rr) {
+            constexpr bool RC = decltype(rr)::value;
+            if (!two_phase) {
+                hipLaunchKernelGGL((k_plain<L, G, true, RC>), dim3((unsigned)grid_for(c, n)),
+                                   dim3(bt), 0, s, c.dev, o, out, n);
+                return launched();
             }
-            return launched();
+            return dispatch_c(c.tun.combine, [&](auto cc) {
+                constexpr bool BC = decltype(cc)::value;
+                for (uint64_t lo = 0; lo < n; lo += c.submax) {
+                    const uint64_t m = std::min<uint64_t>(c.submax, n - lo);
+                    const uint64_t grid = grid_for(c, m);
+                    hipLaunchKernelGGL((k_phase_r<L, G, BC>), dim3((unsigned)grid), dim3(bt), 0,
+                                       s, c.dev, o, out, lo, m);
+                    hipLaunchKernelGGL((k_phase_w<L, G, RC>), dim3((unsigned)grid), dim3(bt), 0,
+                                       s, c.dev, o, lo, m);
+                }
+                return launched();
+            });
         });
     });
 }
@@ -99,14 +109,18 @@ int Launch<L>::batches(const LaunchCtx& c, const OpsDev& o, const OutDev& out,
         constexpr int G = decltype(gc)::value;
         return dispatch_c(c.tun.combine, [&](auto cc) {
             constexpr bool BC = decltype(cc)::value;
-            uint64_t grid = 0;
-            const int rc =
-                persistent_grid(c, reinterpret_cast<const void*>(k_batches<L, G, BC>), &grid);
-            if (rc) return rc;
-            hipLaunchKernelGGL(k_reset_bar<L>, dim3(1), dim3(64), 0, s, bar);
-            hipLaunchKernelGGL((k_batches<L, G, BC>), dim3((unsigned)grid), dim3(PERSIST_THREADS),
-                               0, s, c.dev, o, out, offsets, nbatches, c.submax, bar);
-            return launched();
+            return dispatch_r(c.tun.reclaim, [&](auto rr) {
+                constexpr bool RC = decltype(rr)::value;
+                uint64_t grid = 0;
+                const int rc = persistent_grid(
+                    c, reinterpret_cast<const void*>(k_batches<L, G, BC, RC>), &grid);
+                if (rc) return rc;
+                hipLaunchKernelGGL(k_reset_bar<L>, dim3(1), dim3(64), 0, s, bar);
+                hipLaunchKernelGGL((k_batches<L, G, BC, RC>), dim3((unsigned)grid),
+                                   dim3(PERSIST_THREADS), 0, s, c.dev, o, out, offsets, nbatches,
+                                   c.submax, bar);
+                return launched();
+            });
         });
     });
 }
@@ -118,14 +132,18 @@ int Launch<L>::serve(const LaunchCtx& c, const OpsDev& o, const OutDev& out, con
         constexpr int G = decltype(gc)::value;
         return dispatch_c(c.tun.combine, [&](auto cc) {
             constexpr bool BC = decltype(cc)::value;
-            uint64_t grid = 0;
-            const int rc =
-                persistent_grid(c, reinterpret_cast<const void*>(k_serve<L, G, BC>), &grid);
-            if (rc) return rc;
-            hipLaunchKernelGGL(k_reset_serve<L>, dim3(1), dim3(64), 0, s, ss, bar);
-            hipLaunchKernelGGL((k_serve<L, G, BC>), dim3((unsigned)grid), dim3(PERSIST_THREADS),
-                               0, s, c.dev, o, out, sv, ss, bar, c.submax);
-            return launched();
+            return dispatch_r(c.tun.reclaim, [&](auto rr) {
+                constexpr bool RC = decltype(rr)::value;
+                uint64_t grid = 0;
+                const int rc = persistent_grid(
+                    c, reinterpret_cast<const void*>(k_serve<L, G, BC, RC>), &grid);
+                if (rc) return rc;
+                hipLaunchKernelGGL(k_reset_serve<L>, dim3(1), dim3(64), 0, s, ss, bar);
+                hipLaunchKernelGGL((k_serve<L, G, BC, RC>), dim3((unsigned)grid),
+                                   dim3(PERSIST_THREADS), 0, s, c.dev, o, out, sv, ss, bar,
+                                   c.submax);
+                return launched();
+            });
         });
     });
 }

@@ -179,7 +179,6 @@ static int create_common(lslab_table** out, TableType type, bool is_map, int kb,
     t->dev.status = &t->state->status;
     t->dev.free_head = &t->state->free_head;
     t->dev.free_count = &t->state->free_count;
-    t->dev.reclaim = t->tun.reclaim == LSLAB_RECLAIM_ON;
     t->dev.nbuckets = t->nbuckets;
     t->dev.pool = pool;
     t->dev.mask = (uint32_t)(t->nbuckets - 1);

@@ -78,7 +78,6 @@ struct TableDev {
     uint32_t mask;         // nbuckets - 1
     uint32_t gt_mask;      // gt capacity - 1
     uint32_t hash_mix;     // bucket by mix64(key) instead of the key itself
-    uint32_t reclaim;      // reclaim policy: first free slot of the chain, free emptied pool slabs
     uint64_t* free_head;   // reclaim: free pool slabs, (tag << 32 | slab), 0 = empty
     uint64_t* free_count;  // reclaim: slabs on the free list
 };
