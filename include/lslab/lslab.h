@@ -66,6 +66,16 @@ inline lslab_tuning with_hash(const lslab_tuning* tuning) {
 }
 }  // namespace detail
 
+/// Caller-owned device memory for a table's slab array (bucket heads, then
+/// the pool).  It stands where the reference's Allocator argument supplies
+/// the pool (reference map.h:57, set.h:42): allocate it as you like (at least
+/// map::slab_bytes / set::slab_bytes bytes, 256-B aligned) and keep it alive
+/// until the table is destroyed; the table never frees it.
+struct slab_memory {
+    void* ptr;
+    size_t bytes;
+};
+
 /// Layout of cuda::std::pair<bool, V>: the reference's get() result
 /// (include/lslab/map.h:208-211).
 template <typename V>

@@ -33,6 +33,22 @@ public:
               "map_create");
     }
 
+    /// Table whose slab array lives in caller memory (the reference's
+    /// map(n_log_2, Allocator&&), map.h:57).
+    map(unsigned n_log_2, uint64_t pool_slabs, slab_memory mem,
+        const lslab_tuning* tuning = nullptr)
+        : n_log_2_(n_log_2) {
+        const lslab_tuning tu = detail::with_hash<Hash>(tuning);
+        check(lslab_map_create_at(&t_, sizeof(K), sizeof(V), n_log_2, pool_slabs, mem.ptr,
+                                  mem.bytes, &tu),
+              "map_create_at");
+    }
+
+    /// Bytes of slab_memory a map(n_log_2, pool_slabs, mem) needs.
+    static uint64_t slab_bytes(unsigned n_log_2, uint64_t pool_slabs) {
+        return lslab_slab_bytes(1, sizeof(K), sizeof(V), n_log_2, pool_slabs);
+    }
+
     map(const map&) = delete;
     map& operator=(const map&) = delete;
     map(map&& o) noexcept : t_(o.t_), n_log_2_(o.n_log_2_) { o.t_ = nullptr; }

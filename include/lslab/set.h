@@ -27,6 +27,21 @@ public:
         check(lslab_set_create(&t_, sizeof(K), n_log_2, pool_slabs, &tu), "set_create");
     }
 
+    /// Table whose slab array lives in caller memory (the reference's
+    /// set(n_log_2, Allocator&&), set.h:42).
+    set(unsigned n_log_2, uint64_t pool_slabs, slab_memory mem,
+        const lslab_tuning* tuning = nullptr)
+        : n_log_2_(n_log_2) {
+        const lslab_tuning tu = detail::with_hash<Hash>(tuning);
+        check(lslab_set_create_at(&t_, sizeof(K), n_log_2, pool_slabs, mem.ptr, mem.bytes, &tu),
+              "set_create_at");
+    }
+
+    /// Bytes of slab_memory a set(n_log_2, pool_slabs, mem) needs.
+    static uint64_t slab_bytes(unsigned n_log_2, uint64_t pool_slabs) {
+        return lslab_slab_bytes(0, sizeof(K), 0, n_log_2, pool_slabs);
+    }
+
     set(const set&) = delete;
     set& operator=(const set&) = delete;
     set(set&& o) noexcept : t_(o.t_), n_log_2_(o.n_log_2_) { o.t_ = nullptr; }

@@ -113,6 +113,24 @@ int lslab_map_create(lslab_table** out, int key_bytes, int value_bytes, unsigned
                      uint64_t pool_slabs, const lslab_tuning* tuning);
 int lslab_set_create(lslab_table** out, int key_bytes, unsigned n_log2, uint64_t pool_slabs,
                      const lslab_tuning* tuning);
+/* Pluggable slab memory.  The reference takes an Allocator type that owns the
+ * pool slabs (map.h:24 and map.h:57, set.h:21 and set.h:42; its default is a
+ * device bump pool, device_allocator.h:13-33, and its tests plug in one over
+ * a group allocator, test/slabunified_test.cu:92-162).  Here the caller hands
+ * over the memory of the whole slab array instead (the 2^n bucket heads, then
+ * the pool), allocated however it likes: device-accessible, 256-B aligned, at
+ * least lslab_slab_bytes() bytes, owned by the caller and kept alive until
+ * lslab_destroy (which does not free it).  The library zeroes it.  Locks and
+ * scratch stay library-allocated.  slab_bytes returns 0 for an invalid
+ * shape; create_at returns LSLAB_ERR_INVALID for a misaligned or short
+ * buffer. */
+uint64_t lslab_slab_bytes(int is_map, int key_bytes, int value_bytes, unsigned n_log2,
+                          uint64_t pool_slabs);
+int lslab_map_create_at(lslab_table** out, int key_bytes, int value_bytes, unsigned n_log2,
+                        uint64_t pool_slabs, void* slab_mem, uint64_t slab_mem_bytes,
+                        const lslab_tuning* tuning);
+int lslab_set_create_at(lslab_table** out, int key_bytes, unsigned n_log2, uint64_t pool_slabs,
+                        void* slab_mem, uint64_t slab_mem_bytes, const lslab_tuning* tuning);
 int lslab_destroy(lslab_table* t);
 int lslab_set_tuning(lslab_table* t, const lslab_tuning* tuning);
 int lslab_get_tuning(const lslab_table* t, lslab_tuning* out);

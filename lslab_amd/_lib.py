@@ -41,7 +41,7 @@ EXPORTS = (
     "lslab_map_remove", "lslab_map_mixed", "lslab_set_insert", "lslab_set_contains",
     "lslab_set_remove", "lslab_set_mixed", "lslab_exec_batches", "lslab_status",
     "lslab_stats_get", "lslab_dump", "lslab_error_string", "lslab_serve", "lslab_serve_result",
-    "lslab_chain_histogram",
+    "lslab_chain_histogram", "lslab_slab_bytes", "lslab_map_create_at", "lslab_set_create_at",
 )
 
 SERVE_REJECTED = 0xFFFFFFFF
@@ -136,11 +136,15 @@ def load():
         "lslab_serve": [vp, ctypes.POINTER(ServeParams), vp, vp, vp, vp, vp],
         "lslab_serve_result": [vp, ctypes.POINTER(ServeStats), vp],
         "lslab_chain_histogram": [vp, ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint, vp],
+        "lslab_map_create_at": [pp, i32, i32, ctypes.c_uint, u64, vp, u64, ctypes.POINTER(Tuning)],
+        "lslab_set_create_at": [pp, i32, ctypes.c_uint, u64, vp, u64, ctypes.POINTER(Tuning)],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
         f.argtypes = args
         f.restype = ctypes.c_int
+    lib.lslab_slab_bytes.argtypes = [i32, i32, i32, ctypes.c_uint, u64]
+    lib.lslab_slab_bytes.restype = u64
     lib.lslab_error_string.argtypes = [ctypes.c_int]
     lib.lslab_error_string.restype = ctypes.c_char_p
     _LIB = lib

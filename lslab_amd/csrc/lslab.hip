@@ -25,6 +25,7 @@ struct lslab_table {
     hipEvent_t order_ev;
     int device;
     int num_cus;
+    bool own_slabs;  // false: the slabs live in caller memory (create_at)
 };
 
 #define HIPCHK(x)                                   \
@@ -70,7 +71,7 @@ static bool tuning_ok(const lslab_tuning& tu) {
 
 static void free_table(lslab_table* t) {
     if (!t) return;
-    (void)hipFree(t->dev.slabs);
+    if (t->own_slabs) (void)hipFree(t->dev.slabs);
     (void)hipFree(t->dev.locks);
     (void)hipFree(t->state);
     (void)hipFree(t->dev.gt_key);
@@ -117,11 +118,50 @@ static int alloc_scratch(lslab_table* t) {
     return LSLAB_OK;
 }
 
+static uint64_t layout_slab_bytes(TableType type) {
+    switch (type) {
+        case T_MAP_32_32: return L_M3232::SLAB_BYTES;
+        case T_MAP_32_64: return L_M3264::SLAB_BYTES;
+        case T_MAP_64_32: return L_M6432::SLAB_BYTES;
+        case T_MAP_64_64: return L_M6464::SLAB_BYTES;
+        case T_SET_32: return L_S32::SLAB_BYTES;
+        case T_SET_64: return L_S64::SLAB_BYTES;
+    }
+    return 0;
+}
+
+static bool table_type(bool is_map, int kb, int vb, TableType* ty) {
+    if (is_map) {
+        if (kb == 4 && vb == 4) *ty = T_MAP_32_32;
+        else if (kb == 4 && vb == 8) *ty = T_MAP_32_64;
+        else if (kb == 8 && vb == 4) *ty = T_MAP_64_32;
+        else if (kb == 8 && vb == 8) *ty = T_MAP_64_64;
+        else return false;
+    } else {
+        if (kb == 4) *ty = T_SET_32;
+        else if (kb == 8) *ty = T_SET_64;
+        else return false;
+    }
+    return true;
+}
+
+// Bytes of the slab array (bucket heads, then the pool); 0 if invalid.
+static uint64_t slab_array_bytes(TableType ty, unsigned n_log2, uint64_t pool) {
+    if (n_log2 > 30) return 0;
+    const uint64_t nslabs = (1ull << n_log2) + pool;
+    if (nslabs >= (1ull << 32)) return 0;
+    return nslabs * layout_slab_bytes(ty);
+}
+
 static int create_common(lslab_table** out, TableType type, bool is_map, int kb, int vb,
-                         unsigned n_log2, uint64_t pool, const lslab_tuning* tuning) {
+                         unsigned n_log2, uint64_t pool, const lslab_tuning* tuning,
+                         void* slab_mem = nullptr, uint64_t slab_mem_bytes = 0) {
     if (!out) return LSLAB_ERR_INVALID;
     *out = nullptr;
     if (n_log2 > 30) return LSLAB_ERR_INVALID;
+    if (slab_mem && ((uintptr_t)slab_mem % 256 != 0 ||
+                     slab_mem_bytes < slab_array_bytes(type, n_log2, pool)))
+        return LSLAB_ERR_INVALID;
     lslab_tuning tu{};
     if (tuning) tu = *tuning;
     normalise(tu);
@@ -153,7 +193,9 @@ static int create_common(lslab_table** out, TableType type, bool is_map, int kb,
     }
     int rc = LSLAB_OK;
     const uint64_t slab_total = nslabs * t->slab_bytes;
-    if (hipMalloc(&t->dev.slabs, slab_total) != hipSuccess ||
+    t->own_slabs = slab_mem == nullptr;
+    if (slab_mem) t->dev.slabs = static_cast<char*>(slab_mem);
+    if ((t->own_slabs && hipMalloc(&t->dev.slabs, slab_total) != hipSuccess) ||
         hipMalloc(&t->dev.locks, t->nbuckets * sizeof(uint32_t)) != hipSuccess ||
         hipMalloc(&t->state, sizeof(DevState)) != hipSuccess) {
         free_table(t);
@@ -183,8 +225,9 @@ static int create_common(lslab_table** out, TableType type, bool is_map, int kb,
     t->dev.pool = pool;
     t->dev.mask = (uint32_t)(t->nbuckets - 1);
     t->dev.hash_mix = t->tun.hash == LSLAB_HASH_MIX;
-    t->device_bytes = slab_total + t->nbuckets * 4 + sizeof(DevState) + (t->gt_cap + 1) * 12 +
-                      rep_len(t) * sizeof(RepEnt) + rep_cnt_len(t->submax) * 4;
+    t->device_bytes = (t->own_slabs ? slab_total : 0) + t->nbuckets * 4 + sizeof(DevState) +
+                      (t->gt_cap + 1) * 12 + rep_len(t) * sizeof(RepEnt) +
+                      rep_cnt_len(t->submax) * 4;
     *out = t;
     return LSLAB_OK;
 }
@@ -192,21 +235,40 @@ static int create_common(lslab_table** out, TableType type, bool is_map, int kb,
 extern "C" int lslab_map_create(lslab_table** out, int key_bytes, int value_bytes, unsigned n_log2,
                                 uint64_t pool_slabs, const lslab_tuning* tuning) {
     TableType ty;
-    if (key_bytes == 4 && value_bytes == 4) ty = T_MAP_32_32;
-    else if (key_bytes == 4 && value_bytes == 8) ty = T_MAP_32_64;
-    else if (key_bytes == 8 && value_bytes == 4) ty = T_MAP_64_32;
-    else if (key_bytes == 8 && value_bytes == 8) ty = T_MAP_64_64;
-    else return LSLAB_ERR_INVALID;
+    if (!table_type(true, key_bytes, value_bytes, &ty)) return LSLAB_ERR_INVALID;
     return create_common(out, ty, true, key_bytes, value_bytes, n_log2, pool_slabs, tuning);
 }
 
 extern "C" int lslab_set_create(lslab_table** out, int key_bytes, unsigned n_log2,
                                 uint64_t pool_slabs, const lslab_tuning* tuning) {
     TableType ty;
-    if (key_bytes == 4) ty = T_SET_32;
-    else if (key_bytes == 8) ty = T_SET_64;
-    else return LSLAB_ERR_INVALID;
+    if (!table_type(false, key_bytes, 0, &ty)) return LSLAB_ERR_INVALID;
     return create_common(out, ty, false, key_bytes, 0, n_log2, pool_slabs, tuning);
+}
+
+extern "C" uint64_t lslab_slab_bytes(int is_map, int key_bytes, int value_bytes, unsigned n_log2,
+                                     uint64_t pool_slabs) {
+    TableType ty;
+    if (!table_type(is_map != 0, key_bytes, value_bytes, &ty)) return 0;
+    return slab_array_bytes(ty, n_log2, pool_slabs);
+}
+
+extern "C" int lslab_map_create_at(lslab_table** out, int key_bytes, int value_bytes,
+                                   unsigned n_log2, uint64_t pool_slabs, void* slab_mem,
+                                   uint64_t slab_mem_bytes, const lslab_tuning* tuning) {
+    TableType ty;
+    if (!slab_mem || !table_type(true, key_bytes, value_bytes, &ty)) return LSLAB_ERR_INVALID;
+    return create_common(out, ty, true, key_bytes, value_bytes, n_log2, pool_slabs, tuning,
+                         slab_mem, slab_mem_bytes);
+}
+
+extern "C" int lslab_set_create_at(lslab_table** out, int key_bytes, unsigned n_log2,
+                                   uint64_t pool_slabs, void* slab_mem, uint64_t slab_mem_bytes,
+                                   const lslab_tuning* tuning) {
+    TableType ty;
+    if (!slab_mem || !table_type(false, key_bytes, 0, &ty)) return LSLAB_ERR_INVALID;
+    return create_common(out, ty, false, key_bytes, 0, n_log2, pool_slabs, tuning, slab_mem,
+                         slab_mem_bytes);
 }
 
 extern "C" int lslab_destroy(lslab_table* t) {

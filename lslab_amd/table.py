@@ -42,6 +42,21 @@ def _ops(t, n):
     return t
 
 
+def _mem(t):
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise TypeError("slab_mem must be a CUDA tensor")
+    if not t.is_contiguous():
+        raise ValueError("slab_mem must be contiguous")
+    return ctypes.c_void_p(t.data_ptr()), t.numel() * t.element_size()
+
+
+def slab_bytes(is_map, key_bytes, value_bytes, n_log2, pool_slabs):
+    """Bytes of the slab array (heads + pool) a caller-provided slab_mem must hold
+    (lslab_slab_bytes); 0 for an invalid shape."""
+    return _lib.load().lslab_slab_bytes(int(bool(is_map)), key_bytes, value_bytes, n_log2,
+                                        pool_slabs)
+
+
 def tuning(lanes_per_op=4, engine=_lib.ENGINE_PLAIN, block_threads=256, max_blocks=0,
            max_subbatch=1 << 20, combine=_lib.COMBINE_BLOCK, hash=0, reclaim=0):
     """Launch shapes and table policies (lslab_tuning).  hash and reclaim are
@@ -111,13 +126,24 @@ class Map(_Table):
 
     is_map = True
 
-    def __init__(self, key_bytes=8, value_bytes=8, n_log2=10, pool_slabs=10000, tun=None):
+    def __init__(self, key_bytes=8, value_bytes=8, n_log2=10, pool_slabs=10000, tun=None,
+                 slab_mem=None):
+        """slab_mem: optional caller-owned CUDA tensor holding the slab array (at least
+        slab_bytes(...) bytes, 256-B aligned); the table keeps a reference to it."""
         super().__init__()
         self.kb, self.vb, self.n_log2 = key_bytes, value_bytes, n_log2
         h = ctypes.c_void_p()
-        _lib.check(_lib.load().lslab_map_create(ctypes.byref(h), key_bytes, value_bytes, n_log2,
-                                                pool_slabs, ctypes.byref(tun) if tun else None),
-                   "map_create")
+        tp = ctypes.byref(tun) if tun else None
+        lib = _lib.load()
+        if slab_mem is None:
+            rc = lib.lslab_map_create(ctypes.byref(h), key_bytes, value_bytes, n_log2,
+                                      pool_slabs, tp)
+        else:
+            ptr, nbytes = _mem(slab_mem)
+            rc = lib.lslab_map_create_at(ctypes.byref(h), key_bytes, value_bytes, n_log2,
+                                         pool_slabs, ptr, nbytes, tp)
+        _lib.check(rc, "map_create")
+        self._slab_mem = slab_mem
         self._h = h
 
     def _pairs(self, n):
@@ -240,13 +266,21 @@ class Map(_Table):
 class Set(_Table):
     """lslab::set<K> (reference include/lslab/set.h:21-123)."""
 
-    def __init__(self, key_bytes=8, n_log2=10, pool_slabs=10000, tun=None):
+    def __init__(self, key_bytes=8, n_log2=10, pool_slabs=10000, tun=None, slab_mem=None):
+        """slab_mem: as for Map."""
         super().__init__()
         self.kb, self.n_log2 = key_bytes, n_log2
         h = ctypes.c_void_p()
-        _lib.check(_lib.load().lslab_set_create(ctypes.byref(h), key_bytes, n_log2, pool_slabs,
-                                                ctypes.byref(tun) if tun else None),
-                   "set_create")
+        tp = ctypes.byref(tun) if tun else None
+        lib = _lib.load()
+        if slab_mem is None:
+            rc = lib.lslab_set_create(ctypes.byref(h), key_bytes, n_log2, pool_slabs, tp)
+        else:
+            ptr, nbytes = _mem(slab_mem)
+            rc = lib.lslab_set_create_at(ctypes.byref(h), key_bytes, n_log2, pool_slabs, ptr,
+                                         nbytes, tp)
+        _lib.check(rc, "set_create")
+        self._slab_mem = slab_mem
         self._h = h
 
     def _call(self, fn, keys, stream):

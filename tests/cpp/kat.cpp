@@ -1,7 +1,8 @@
 // The reference's known-answer tests, restated over the header-only C++ API:
 //   GetPutTest                 test/slabunified_test.cu:42-89
-//   GetPutTestGroupAllocator   test/slabunified_test.cu:156-205 (a pool of a
-//                              different size stands in for the foreign allocator)
+//   GetPutTestGroupAllocator   test/slabunified_test.cu:156-205 (the foreign
+//                              allocator is a caller-owned arena: two maps'
+//                              slab arrays carved from one hipMalloc)
 //   ContainsInsertRemoveTest   test/set_test.cu:65-106
 // GetPutTest also runs on a table created with lslab::mix_hash (the
 // reference's Hash template parameter with a non-identity functor).
@@ -45,11 +46,8 @@ struct dvec {
 };
 
 template <typename Hash = lslab::hash<int>>
-static void get_put(uint64_t pool, int engine) {
+static void get_put_on(lslab::map<int, int, Hash>& m) {
     constexpr int size = 1024;
-    lslab_tuning tu{};
-    tu.engine = engine;
-    lslab::map<int, int, Hash> m(10, pool, &tu);
     std::vector<int> hk(size);
     for (int i = 0; i < size; ++i) hk[i] = i + 1;  // thrust::sequence(.., 1)
     dvec<int> keys(size), values(size), res1(size);
@@ -70,6 +68,51 @@ static void get_put(uint64_t pool, int engine) {
                    "get result error key " + std::to_string(i + 1) + " rep " + std::to_string(rep));
     }
     ASSERT(m.status() == LSLAB_OK, "device status");
+}
+
+template <typename Hash = lslab::hash<int>>
+static void get_put(uint64_t pool, int engine) {
+    lslab_tuning tu{};
+    tu.engine = engine;
+    lslab::map<int, int, Hash> m(10, pool, &tu);
+    get_put_on(m);
+}
+
+// Two maps whose slab arrays share one caller arena; the first keeps its
+// contents while the second runs GetPutTest, and a short buffer is refused.
+static void get_put_group(uint64_t pool, int engine) {
+    using map_t = lslab::map<int, int>;
+    lslab_tuning tu{};
+    tu.engine = engine;
+    const uint64_t need = map_t::slab_bytes(10, pool);
+    ASSERT(need == ((1ull << 10) + pool) * 256, "slab_bytes");
+    const uint64_t stride = (need + 255) / 256 * 256;
+    dvec<uint8_t> arena(2 * stride);
+    bool refused = false;
+    try {
+        map_t bad(10, pool, lslab::slab_memory{arena.p, need - 1}, &tu);
+    } catch (const lslab::error&) {
+        refused = true;
+    }
+    ASSERT(refused, "short slab memory accepted");
+    map_t other(10, pool, lslab::slab_memory{arena.p, stride}, &tu);
+    constexpr int size = 1024;
+    std::vector<int> hk(size), hv(size);
+    for (int i = 0; i < size; ++i) hk[i] = -(i + 1), hv[i] = 7 * i;
+    dvec<int> keys(size), values(size), old(size);
+    dvec<lslab::found_value<int>> res(size);
+    keys.from(hk);
+    values.from(hv);
+    other.put(keys.p, values.p, old.p, size);
+    {
+        map_t m(10, pool, lslab::slab_memory{arena.p + stride, stride}, &tu);
+        get_put_on(m);
+    }
+    other.get(keys.p, res.p, size);
+    HIPOK(hipDeviceSynchronize());
+    auto r = res.to();
+    for (int i = 0; i < size; ++i) ASSERT(r[i].first && r[i].second == 7 * i, "arena neighbour");
+    ASSERT(other.status() == LSLAB_OK, "device status");
 }
 
 static void contains_insert_remove(int engine) {
@@ -100,7 +143,7 @@ int main() {
         for (int engine = 0; engine < 2; ++engine) {
             get_put(10000, engine);
             std::printf("GetPutTest engine %d ok\n", engine);
-            get_put(2097152 / 256, engine);
+            get_put_group(10000, engine);
             std::printf("GetPutTestGroupAllocator engine %d ok\n", engine);
             contains_insert_remove(engine);
             std::printf("ContainsInsertRemoveTest engine %d ok\n", engine);

@@ -68,3 +68,19 @@ def test_cpp_headers_compile():
                             "-I", os.path.join(ROOT, "include"), "-I", "/opt/rocm/include", f],
                            capture_output=True, text=True)
         assert r.returncode == 0, r.stderr
+
+
+def test_slab_bytes_closed_form(lib):
+    """lslab_slab_bytes: (2^n + pool) slabs of the layout's size; 0 for an
+    invalid shape.  Pure arithmetic, no device call."""
+    f = lib.lslab_slab_bytes
+    assert f(1, 8, 8, 10, 100) == (1024 + 100) * 512
+    assert f(1, 4, 4, 3, 0) == 8 * 256
+    assert f(1, 4, 8, 3, 5) == 13 * 384
+    assert f(1, 8, 4, 3, 5) == 13 * 384
+    assert f(0, 8, 0, 4, 1) == 17 * 256
+    assert f(0, 4, 0, 4, 1) == 17 * 128
+    assert f(1, 2, 8, 4, 1) == 0
+    assert f(0, 16, 0, 4, 1) == 0
+    assert f(1, 8, 8, 31, 1) == 0
+    assert f(1, 8, 8, 30, 1 << 32) == 0

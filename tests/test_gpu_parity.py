@@ -509,3 +509,58 @@ def test_reclaim_random_stream(lslab, oracle_mod, is_map, kb, vb, engine, lanes)
     st = tbl.stats()
     assert st["present"] == 0 and st["slabs_used"] == 0
     assert tbl.chain_histogram(4) == [1 << n_log2, 0, 0, 0]
+
+
+@pytest.mark.parametrize("is_map,kb,vb", [(True, 8, 8), (True, 4, 8), (False, 4, 0)])
+@pytest.mark.parametrize("engine", ENGINES)
+def test_caller_slab_memory(lslab, oracle_mod, is_map, kb, vb, engine):
+    """A table whose slab array lives in a caller-owned buffer (the reference's
+    Allocator argument): the library zeroes the garbage it is given, results are
+    the replay's, a short or misaligned buffer is refused, the bytes are the
+    table's, and destroying the table leaves the buffer to its owner."""
+    say("caller_slab_memory", is_map, kb, vb, engine)
+    n_log2, pool = 6, 1 << 10
+    cap = 30 if (not is_map and kb == 4) else 31
+    need = lslab.slab_bytes(is_map, kb, vb, n_log2, pool)
+    slab = {(True, 8, 8): 512, (True, 4, 8): 384, (False, 4, 0): 128}[(is_map, kb, vb)]
+    assert need == ((1 << n_log2) + pool) * slab
+    arena = torch.full((need + 512,), 0xAB, dtype=torch.uint8, device="cuda")
+    tun = lslab.tuning(lanes_per_op=8, engine=engine, max_subbatch=1 << 14)
+
+    def mk(mem):
+        return (lslab.Map(kb, vb, n_log2, pool, tun, slab_mem=mem) if is_map
+                else lslab.Set(kb, n_log2, pool, tun, slab_mem=mem))
+
+    with pytest.raises(lslab.LslabError):
+        mk(arena[:need - 1])
+    with pytest.raises(lslab.LslabError):
+        mk(arena[1:need + 1])
+    tbl = mk(arena[256:256 + need])
+    ref = Replay(is_map, kb, vb, n_log2, pool_slabs=pool, slab_keys=cap)
+    rng = np.random.default_rng(11)
+    for step in range(6):
+        keys = rng.integers(1, 20000, size=5000, dtype=np.uint64)
+        ops = (np.full(len(keys), OP_PUT, np.uint8) if step == 0 else
+               rng.choice([OP_GET, OP_PUT, OP_REMOVE], size=len(keys)).astype(np.uint8))
+        keys = np.unique(keys)
+        ops = ops[:len(keys)]
+        vals = rng.integers(1, 2**31, size=len(keys), dtype=np.uint64)
+        f, v = run_batch(tbl, is_map, kb, vb, ops, keys, vals)
+        rf, rv, fail = ref.apply(ops, keys, vals if is_map else None)
+        assert fail == 0
+        np.testing.assert_array_equal(f, rf)
+        np.testing.assert_array_equal(v, rv)
+    assert tbl.status() == 0
+    k, vv = dump_sorted(tbl, is_map, kb, vb)
+    rk, rvv = oracle_dump_sorted(ref)
+    np.testing.assert_array_equal(k, rk)
+    np.testing.assert_array_equal(vv, rvv)
+    torch.cuda.synchronize()
+    host = arena.cpu()
+    # bytes outside the table's range are untouched; inside, the table wrote
+    assert (host[:256] == 0xAB).all() and (host[256 + need:] == 0xAB).all()
+    assert not (host[256:256 + need] == 0xAB).all()
+    tbl.close()
+    arena.fill_(1)
+    torch.cuda.synchronize()
+    assert int(arena.sum()) == need + 512
