@@ -41,8 +41,11 @@ extern "C" {
 #define LSLAB_ERR_CORRUPT 6        /* a chain link or batch index failed its bounds check */
 
 /* Opcodes of the mixed-op entry points.  Maps: get/put/remove; sets:
- * contains/insert/remove.  Same numbering as the legacy per-request types of
- * the reference (test/slabunified_test.cu.old:60-67), minus EMPTY. */
+ * contains/insert/remove.  The reference's legacy tests tag requests the same
+ * way (REQUEST_INSERT / REQUEST_REMOVE / REQUEST_EMPTY,
+ * test/slabunified_test.cu.old:60-67) without the repository defining their
+ * values, so the numbering is this ABI's own.  Any other opcode value is
+ * treated as GET. */
 #define LSLAB_OP_GET 0
 #define LSLAB_OP_PUT 1
 #define LSLAB_OP_REMOVE 2
@@ -231,6 +234,55 @@ int lslab_stats_get(lslab_table* t, lslab_stats* out, hipStream_t stream);
 int lslab_chain_histogram(lslab_table* t, uint64_t* hist, unsigned nbins, hipStream_t stream);
 int lslab_dump(lslab_table* t, void* keys_out, void* vals_out, uint64_t cap, uint64_t* count,
                hipStream_t stream);
+
+/* Shape of a table (new): 1 for a map, key and value widths in bytes (0 for
+ * a set's value), and the HIP device it lives on. */
+int lslab_table_info(const lslab_table* t, int* is_map, int* key_bytes, int* value_bytes,
+                     int* device);
+
+/* KVCG hot/cold split (new).  KVCG, the system the reference's table comes
+ * from (README.md:3, CITATION.bib:1-17), keeps the hottest keys of a skewed
+ * workload in a CPU-side cache in front of the GPU slab table, for requests
+ * that arrive in host memory; the reference repository holds only the GPU
+ * table.  Here a lslab_kvcg owns a host tier of up to hot_capacity keys in
+ * front of table t:
+ *   - the hot set is chosen by the caller (set_hot) or as the k most frequent
+ *     keys of a sample (sample_hot; ties go to the smaller key).  While a key
+ *     is hot the host tier is its only authority; demoting it writes its
+ *     state back to the table (put if present, remove if absent), and flush
+ *     writes back every hot key (the table is then complete, e.g. for
+ *     lslab_dump).
+ *   - mixed runs a batch whose ops, keys, values and out are HOST memory
+ *     (out laid out as for lslab_map_mixed / lslab_set_mixed): ops on hot
+ *     keys run on `threads` host workers in batch order while the other ops
+ *     are staged through pinned memory and run on the table in one mixed call
+ *     on `stream`.  Per key the results follow a serial order (batch order
+ *     for hot keys, the table engine's order for cold ones); a batch of
+ *     distinct keys gives exactly the sequential replay's results.  Batches
+ *     larger than max_batch run as consecutive slices.  The call returns
+ *     when the batch is complete.
+ *   - destroy flushes on the null stream, then frees; the table stays.
+ * threads 0 = min(16, hardware threads); max_batch 0 = 2^20.  The table must
+ * not be used directly while a kvcg holds hot keys of it, except after a
+ * flush and before the next kvcg call. */
+typedef struct lslab_kvcg lslab_kvcg;
+typedef struct {
+    uint64_t hot_keys;     /* keys in the hot set */
+    uint64_t hot_capacity;
+    uint64_t hot_ops;      /* ops run by the host tier */
+    uint64_t cold_ops;     /* ops sent to the table */
+    uint64_t batches;      /* mixed calls */
+} lslab_kvcg_stats;
+int lslab_kvcg_create(lslab_kvcg** out, lslab_table* t, uint64_t hot_capacity,
+                      uint64_t max_batch, int threads);
+int lslab_kvcg_destroy(lslab_kvcg* kv);
+int lslab_kvcg_set_hot(lslab_kvcg* kv, const void* keys, uint64_t n, hipStream_t stream);
+int lslab_kvcg_sample_hot(lslab_kvcg* kv, const void* sample_keys, uint64_t n, uint64_t k,
+                          hipStream_t stream);
+int lslab_kvcg_mixed(lslab_kvcg* kv, const uint8_t* ops, const void* keys, const void* values,
+                     void* out, uint64_t n, hipStream_t stream);
+int lslab_kvcg_flush(lslab_kvcg* kv, hipStream_t stream);
+int lslab_kvcg_stats_get(const lslab_kvcg* kv, lslab_kvcg_stats* out);
 
 const char* lslab_error_string(int status);
 

@@ -42,6 +42,8 @@ EXPORTS = (
     "lslab_set_remove", "lslab_set_mixed", "lslab_exec_batches", "lslab_status",
     "lslab_stats_get", "lslab_dump", "lslab_error_string", "lslab_serve", "lslab_serve_result",
     "lslab_chain_histogram", "lslab_slab_bytes", "lslab_map_create_at", "lslab_set_create_at",
+    "lslab_table_info", "lslab_kvcg_create", "lslab_kvcg_destroy", "lslab_kvcg_set_hot",
+    "lslab_kvcg_sample_hot", "lslab_kvcg_mixed", "lslab_kvcg_flush", "lslab_kvcg_stats_get",
 )
 
 SERVE_REJECTED = 0xFFFFFFFF
@@ -94,6 +96,16 @@ class ServeStats(ctypes.Structure):
     ]
 
 
+class KvcgStats(ctypes.Structure):
+    _fields_ = [
+        ("hot_keys", ctypes.c_uint64),
+        ("hot_capacity", ctypes.c_uint64),
+        ("hot_ops", ctypes.c_uint64),
+        ("cold_ops", ctypes.c_uint64),
+        ("batches", ctypes.c_uint64),
+    ]
+
+
 class LslabError(RuntimeError):
     def __init__(self, code, what=""):
         self.code = code
@@ -138,6 +150,14 @@ def load():
         "lslab_chain_histogram": [vp, ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint, vp],
         "lslab_map_create_at": [pp, i32, i32, ctypes.c_uint, u64, vp, u64, ctypes.POINTER(Tuning)],
         "lslab_set_create_at": [pp, i32, ctypes.c_uint, u64, vp, u64, ctypes.POINTER(Tuning)],
+        "lslab_table_info": [vp] + [ctypes.POINTER(ctypes.c_int)] * 4,
+        "lslab_kvcg_create": [pp, vp, u64, u64, i32],
+        "lslab_kvcg_destroy": [vp],
+        "lslab_kvcg_set_hot": [vp, vp, u64, vp],
+        "lslab_kvcg_sample_hot": [vp, vp, u64, u64, vp],
+        "lslab_kvcg_mixed": [vp, vp, vp, vp, vp, u64, vp],
+        "lslab_kvcg_flush": [vp, vp],
+        "lslab_kvcg_stats_get": [vp, ctypes.POINTER(KvcgStats)],
     }
     for name, args in sig.items():
         f = getattr(lib, name)

@@ -2,8 +2,8 @@
 
 The kernels of the six table layouts are separate translation units
 (csrc/launch.hip compiled once per layout), compiled in parallel with the C
-ABI (csrc/lslab.hip) and the workload generators, then linked into one
-shared library.
+ABI (csrc/lslab.hip), the workload generators and the host-side KVCG split
+(csrc/kvcg.cpp), then linked into one shared library.
 
 python -m lslab_amd.build [--verbose]
 """
@@ -16,7 +16,8 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LAYOUTS = ["L_M3232", "L_M3264", "L_M6432", "L_M6464", "L_S32", "L_S64"]
-UNITS = ([("lslab", "csrc/lslab.hip", []), ("workload", "csrc/workload.hip", [])] +
+UNITS = ([("lslab", "csrc/lslab.hip", []), ("workload", "csrc/workload.hip", []),
+          ("kvcg", "csrc/kvcg.cpp", [])] +
          [(f"launch_{l.lower()}", "csrc/launch.hip", [f"-DLSLAB_LAYOUT={l}"]) for l in LAYOUTS])
 HEADERS = ["csrc/slab.hpp", "csrc/engine.hpp", "csrc/kernels.hpp"]
 OUT = os.path.join(HERE, "liblslab.so")
@@ -55,7 +56,8 @@ def build(verbose=False, jobs=None):
             for f in [ex.submit(_run, c, verbose) for c in cmds]:
                 f.result()
         tmp = OUT + ".tmp"
-        _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs, verbose)
+        _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", tmp] + objs,
+             verbose)
         os.replace(tmp, OUT)
     return OUT
 

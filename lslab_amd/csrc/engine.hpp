@@ -24,13 +24,20 @@
 
 namespace lslab_dev {
 
+// Opcode of op i.  A value outside GET/PUT/REMOVE reads as GET, so no caller
+// opcode reaches the internal ones.
+__device__ __forceinline__ uint8_t ld_opcode(const OpsDev& o, uint64_t i) {
+    const uint8_t raw = o.ops ? o.ops[i] : o.uop;
+    return raw <= OP_REMOVE ? raw : (uint8_t)OP_GET;
+}
+
 // Loads op i of a stream (opcode, key, value).
 template <class L>
 __device__ __forceinline__ void load_op(const OpsDev& o, uint64_t i, uint8_t& op,
                                         typename L::K& key, typename L::V& val) {
     using K = typename L::K;
     using V = typename L::V;
-    op = o.ops ? o.ops[i] : o.uop;
+    op = ld_opcode(o, i);
     key = ld_stride<K>(o.keys, o.kstride, i);
     val = (L::IS_MAP && o.vals && op == OP_PUT) ? ld_stride<V>(o.vals, o.vstride, i) : V(0);
 }
@@ -494,7 +501,7 @@ __device__ __forceinline__ void mut_result(const TableDev& t, const OpsDev& o, c
     }
     if (pred != 0) {
         const uint64_t pi = lo + pred - 1u;
-        const uint8_t pop = o.ops ? o.ops[pi] : o.uop;
+        const uint8_t pop = ld_opcode(o, pi);
         if (pop == OP_PUT) {
             rf = true;
             rv = L::IS_MAP ? ld_stride<V>(o.vals, o.vstride, pi) : V(0);
@@ -596,7 +603,7 @@ __device__ __forceinline__ void phase_r_op(const TableDev& t, const OpsDev& o, c
                 uint32_t P;
                 g = gt_exchange(t, kk, li + 1u, P);
                 uint8_t fop = OP_PUT;
-                if constexpr (!L::IS_MAP) fop = o.ops ? o.ops[lo + fl] : o.uop;
+                if constexpr (!L::IS_MAP) fop = ld_opcode(o, lo + fl);
                 mut_result<L>(t, o, out, lo, m, fl, fop, P, found, cur);
             }
         }

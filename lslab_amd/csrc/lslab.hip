@@ -246,6 +246,16 @@ extern "C" int lslab_set_create(lslab_table** out, int key_bytes, unsigned n_log
     return create_common(out, ty, false, key_bytes, 0, n_log2, pool_slabs, tuning);
 }
 
+extern "C" int lslab_table_info(const lslab_table* t, int* is_map, int* key_bytes,
+                                int* value_bytes, int* device) {
+    if (!t) return LSLAB_ERR_INVALID;
+    if (is_map) *is_map = t->is_map ? 1 : 0;
+    if (key_bytes) *key_bytes = t->key_bytes;
+    if (value_bytes) *value_bytes = t->is_map ? t->val_bytes : 0;
+    if (device) *device = t->device;
+    return LSLAB_OK;
+}
+
 extern "C" uint64_t lslab_slab_bytes(int is_map, int key_bytes, int value_bytes, unsigned n_log2,
                                      uint64_t pool_slabs) {
     TableType ty;

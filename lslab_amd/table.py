@@ -329,3 +329,78 @@ class Set(_Table):
         _lib.check(_lib.load().lslab_dump(self._h, _ptr(k), None, cap, ctypes.byref(c),
                                           _stream(stream)), "dump")
         return k[: int(c.value)]
+
+
+class Kvcg:
+    """KVCG hot/cold split over a Map or Set (lslab_kvcg_*): host-memory batches,
+    hot keys served by host workers, the rest by the table.  Arrays are numpy
+    (host); keys/values as uint64 bit patterns of the table's widths."""
+
+    def __init__(self, table, hot_capacity, max_batch=1 << 20, threads=0):
+        import numpy as np
+        self._np = np
+        self.table = table
+        self.is_map = table.is_map
+        self.kb = table.kb
+        self.vb = table.vb if table.is_map else 0
+        self._kdt = np.uint32 if self.kb == 4 else np.uint64
+        self._vdt = np.uint32 if self.vb == 4 else np.uint64
+        h = ctypes.c_void_p()
+        _lib.check(_lib.load().lslab_kvcg_create(ctypes.byref(h), table._h, hot_capacity,
+                                                 max_batch, threads), "kvcg_create")
+        self._h = h
+
+    def _host(self, a, dt):
+        a = self._np.ascontiguousarray(self._np.asarray(a).astype(self._np.uint64).astype(dt))
+        return a, ctypes.c_void_p(a.ctypes.data)
+
+    def set_hot(self, keys, stream=None):
+        k, kp = self._host(keys, self._kdt)
+        _lib.check(_lib.load().lslab_kvcg_set_hot(self._h, kp, len(k), _stream(stream)),
+                   "kvcg_set_hot")
+
+    def sample_hot(self, sample_keys, k, stream=None):
+        s, sp = self._host(sample_keys, self._kdt)
+        _lib.check(_lib.load().lslab_kvcg_sample_hot(self._h, sp, len(s), k, _stream(stream)),
+                   "kvcg_sample_hot")
+
+    def mixed(self, ops, keys, values=None, stream=None):
+        """Returns (found u8[n], value u64[n]) as numpy arrays (value 0 for sets)."""
+        np = self._np
+        n = len(keys)
+        o = np.ascontiguousarray(np.asarray(ops, dtype=np.uint8))
+        if len(o) != n:
+            raise ValueError("ops and keys differ in length")
+        k, kp = self._host(keys, self._kdt)
+        if self.is_map:
+            v, vp = self._host(values if values is not None else np.zeros(n), self._vdt)
+            out = np.zeros((n, 2), dtype=self._vdt)
+        else:
+            vp = None
+            out = np.zeros(n, dtype=np.uint8)
+        _lib.check(_lib.load().lslab_kvcg_mixed(self._h, ctypes.c_void_p(o.ctypes.data), kp, vp,
+                                                ctypes.c_void_p(out.ctypes.data), n,
+                                                _stream(stream)), "kvcg_mixed")
+        if self.is_map:
+            found = out.view(np.uint8).reshape(n, 2 * self.vb)[:, 0].copy()
+            return found, out[:, 1].astype(np.uint64)
+        return out, np.zeros(n, np.uint64)
+
+    def flush(self, stream=None):
+        _lib.check(_lib.load().lslab_kvcg_flush(self._h, _stream(stream)), "kvcg_flush")
+
+    def stats(self):
+        s = _lib.KvcgStats()
+        _lib.check(_lib.load().lslab_kvcg_stats_get(self._h, ctypes.byref(s)), "kvcg_stats")
+        return {f: getattr(s, f) for f, _ in _lib.KvcgStats._fields_}
+
+    def close(self):
+        if self._h:
+            _lib.check(_lib.load().lslab_kvcg_destroy(self._h), "kvcg_destroy")
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -84,3 +84,17 @@ def test_slab_bytes_closed_form(lib):
     assert f(0, 16, 0, 4, 1) == 0
     assert f(1, 8, 8, 31, 1) == 0
     assert f(1, 8, 8, 30, 1 << 32) == 0
+
+
+def test_kvcg_and_info_reject_null_table(lib):
+    """The KVCG and table-info entry points validate their handle before any
+    device work."""
+    import ctypes
+    h = ctypes.c_void_p()
+    assert lib.lslab_kvcg_create(ctypes.byref(h), None, 16, 0, 1) == 1
+    assert lib.lslab_kvcg_create(None, None, 16, 0, 1) == 1
+    assert lib.lslab_kvcg_destroy(None) == 1
+    assert lib.lslab_kvcg_mixed(None, None, None, None, None, 0, None) == 1
+    assert lib.lslab_kvcg_flush(None, None) == 1
+    i = ctypes.c_int()
+    assert lib.lslab_table_info(None, ctypes.byref(i), None, None, None) == 1
