@@ -13,10 +13,12 @@ resident in HBM before timing starts.
   batches).  value = fused Mops/s.
 
 The JSON line also carries the plain rate, the roofline of the fused kernel
-(algorithmic bytes / HIP-event kernel time), the batched form (4096 uneven
-batches, one graph-captured launch vs one launch per batch), the served form
-(device-resident request loop: capacity, then p50/p99 at 70% of it) and the
-host-core baseline (the oracle's sequential replay of a bounded sample).
+(algorithmic bytes / HIP-event kernel time), the same batches arriving in
+host memory (the table behind PCIe copies vs the KVCG hot/cold split), the batched
+form (4096 uneven batches, one graph-captured launch vs one launch per batch),
+the served form (device-resident request loop: capacity, then p50/p99 at 70%
+of it) and the host-core baseline (the oracle's sequential replay of a bounded
+sample).
 Launch shapes come from lslab_amd/tuning.json (python -m lslab_amd.autotune).
 """
 import argparse
@@ -264,6 +266,98 @@ def batched(lslab_amd, wl, torch, m, tun, nbatches=4096):
             "combine": tun["combine"]}
 
 
+def host_input(lslab_amd, torch, m, tun, ops, keys, vals, nb, hot_keys=1 << 16):
+    """The headline batches arriving in HOST memory (the KVCG setting): (1) the
+    table alone, each batch copied in from pinned memory, run and copied back
+    on one stream (two- and three-stream pipelines measured no faster: the
+    copies did not overlap), (2) the KVCG hot/cold split with the hot_keys most
+    frequent keys of the first batch on host workers, (3) the split with an
+    empty hot set (its staging alone)."""
+    import numpy as np
+    from lslab_amd import _lib
+    lib = _lib.load()
+    n = nb * BATCH
+    m.set_tuning(lslab_amd.tuning(lanes_per_op=tun["lanes_per_op"],
+                                  engine=lslab_amd.ENGINE_TWO_PHASE, max_blocks=tun["max_blocks"],
+                                  max_subbatch=BATCH, combine=tun["combine"]))
+    p_ops = torch.empty(n, dtype=torch.uint8, pin_memory=True)
+    p_keys = torch.empty(n, dtype=torch.int64, pin_memory=True)
+    p_vals = torch.empty(n, dtype=torch.int64, pin_memory=True)
+    p_out = torch.empty((n, 2), dtype=torch.int64, pin_memory=True)
+    p_ops.copy_(ops[:n])
+    p_keys.copy_(keys[:n])
+    p_vals.copy_(vals[:n])
+    d_ops = torch.empty(BATCH, dtype=torch.uint8, device="cuda")
+    d_keys = torch.empty(BATCH, dtype=torch.int64, device="cuda")
+    d_vals = torch.empty(BATCH, dtype=torch.int64, device="cuda")
+    d_out = torch.empty((BATCH, 2), dtype=torch.int64, device="cuda")
+    sp = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+
+    def serial(nbat):
+        for b in range(nbat):
+            sl = slice(b * BATCH, (b + 1) * BATCH)
+            d_ops.copy_(p_ops[sl], non_blocking=True)
+            d_keys.copy_(p_keys[sl], non_blocking=True)
+            d_vals.copy_(p_vals[sl], non_blocking=True)
+            _lib.check(lib.lslab_map_mixed(m.handle, ctypes.c_void_p(d_ops.data_ptr()),
+                                           ctypes.c_void_p(d_keys.data_ptr()),
+                                           ctypes.c_void_p(d_vals.data_ptr()),
+                                           ctypes.c_void_p(d_out.data_ptr()), BATCH, sp),
+                       "map_mixed")
+            p_out[sl].copy_(d_out, non_blocking=True)
+        torch.cuda.synchronize()
+
+    serial(2)
+    t0 = time.perf_counter()
+    serial(nb)
+    pipe_s = time.perf_counter() - t0
+    # raw PCIe rate of pinned copies, both directions
+    big = torch.empty(1 << 26, dtype=torch.int64, pin_memory=True)
+    dbig = torch.empty(1 << 26, dtype=torch.int64, device="cuda")
+    dbig.copy_(big)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dbig.copy_(big, non_blocking=True)
+    torch.cuda.synchronize()
+    h2d = big.numel() * 8 / (time.perf_counter() - t0) / 1e9
+    t0 = time.perf_counter()
+    big.copy_(dbig, non_blocking=True)
+    torch.cuda.synchronize()
+    d2h = big.numel() * 8 / (time.perf_counter() - t0) / 1e9
+    del big, dbig
+    h_ops = np.ascontiguousarray(p_ops.numpy())
+    h_keys = np.ascontiguousarray(p_keys.numpy().view(np.uint64))
+    h_vals = np.ascontiguousarray(p_vals.numpy().view(np.uint64))
+    h_out = np.empty((BATCH, 2), dtype=np.uint64)
+    threads = min(16, os.cpu_count() or 1)
+    res = {"batches": nb, "ops": n, "pcie_h2d_gbs": round(h2d, 1), "pcie_d2h_gbs": round(d2h, 1),
+           "table_via_pcie_mops": round(n / pipe_s / 1e6, 2)}
+    for label, k in (("kvcg", hot_keys), ("kvcg_cold_only", 0)):
+        kv = lslab_amd.Kvcg(m, hot_capacity=max(k, 1), max_batch=BATCH, threads=threads)
+        if k:
+            kv.sample_hot(h_keys[:BATCH], k)
+        kv.mixed(h_ops[:BATCH], h_keys[:BATCH], h_vals[:BATCH], out=h_out, raw=True)  # warm
+        st0 = kv.stats()
+        t0 = time.perf_counter()
+        for b in range(nb):
+            sl = slice(b * BATCH, (b + 1) * BATCH)
+            kv.mixed(h_ops[sl], h_keys[sl], h_vals[sl], out=h_out, raw=True)
+        dt = time.perf_counter() - t0
+        st = kv.stats()
+        hot = st["hot_ops"] - st0["hot_ops"]
+        res[label + "_mops"] = round(n / dt / 1e6, 2)
+        if k:
+            res["hot_keys"] = k
+            res["hot_fraction"] = round(hot / n, 4)
+        kv.close()
+    res["threads"] = threads
+    st = m.status()
+    if st:
+        raise RuntimeError(f"host-input status {st}")
+    return res
+
+
 def cpu_baseline(ops, keys, vals, nb_sample):
     """The oracle's sequential replay of nb_sample batches after the same prefill."""
     import numpy as np
@@ -348,6 +442,7 @@ def main():
     ap.add_argument("--cpu-batches", type=int, default=2)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--no-host-input", action="store_true")
     ap.add_argument("--served-requests", type=int, default=20000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 1 GiB device copy first, as the HBM byte-counter reference")
@@ -455,6 +550,11 @@ def main():
             "bytes_per_op": B_OP,
         },
     }
+    if not args.no_host_input:
+        log("host-memory batches: table over PCIe vs KVCG hot/cold split ...")
+        res["host_input"] = host_input(lslab_amd, torch, m, th, ops[W * BATCH:],
+                                       keys[W * BATCH:], vals[W * BATCH:], min(K, 8))
+        log(res["host_input"])
     if not args.no_batched:
         log("batched form (4096 uneven batches) ...")
         res["batched"] = batched(lslab_amd, wl, torch, m, tuned["batched"])

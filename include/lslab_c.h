@@ -121,7 +121,7 @@ int lslab_set_create(lslab_table** out, int key_bytes, unsigned n_log2, uint64_t
  * device bump pool, device_allocator.h:13-33, and its tests plug in one over
  * a group allocator, test/slabunified_test.cu:92-162).  Here the caller hands
  * over the memory of the whole slab array instead (the 2^n bucket heads, then
- * the pool), allocated however it likes: device-accessible, 256-B aligned, at
+ * the pool), allocated however it likes: device memory, 256-B aligned, at
  * least lslab_slab_bytes() bytes, owned by the caller and kept alive until
  * lslab_destroy (which does not free it).  The library zeroes it.  Locks and
  * scratch stay library-allocated.  slab_bytes returns 0 for an invalid

@@ -130,43 +130,65 @@ private:
     bool stop_ = false;
 };
 
-// The hot keys: dense slots (key, present, value) and an open-addressing
-// index key -> slot at load factor <= 1/2.
-struct HotTier {
-    std::vector<uint64_t> key, val;
-    std::vector<uint8_t> present;
-    std::vector<uint64_t> ikey;
-    std::vector<uint32_t> islot;  // slot + 1; 0 = empty
-    uint64_t imask = 0;
+// State of one hot key (or one op's result): value and presence.
+struct Cell {
+    uint64_t val;
+    uint64_t present;
+};
 
-    void reset(uint64_t cap) {
+// The hot keys: dense slots and an open-addressing index key -> slot at load
+// factor <= 1/2, fronted by a bitmap of 16 bits per index entry that turns
+// away most keys that are not hot before they touch the index.  Slot s
+// belongs to shard s % shards (the worker that runs its ops), and its state
+// lives in that shard's own array, so workers never write to a shared cache
+// line.
+struct HotTier {
+    struct Ent {
+        uint64_t key;
+        uint64_t slot;  // slot + 1; 0 = empty
+    };
+    std::vector<uint64_t> key;
+    std::vector<std::vector<Cell>> cell;  // [shard][slot / shards]
+    std::vector<Ent> index;
+    std::vector<uint64_t> filter;
+    uint64_t imask = 0, fmask = 0;
+    int shards = 1;
+
+    void reset(uint64_t cap, int nshards) {
         key.clear();
-        val.clear();
-        present.clear();
+        shards = nshards;
+        cell.assign(nshards, {});
         uint64_t ic = 2;
         while (ic < 2 * cap) ic <<= 1;
-        ikey.assign(ic, 0);
-        islot.assign(ic, 0);
+        index.assign(ic, Ent{0, 0});
         imask = ic - 1;
+        filter.assign(std::max<uint64_t>(ic / 4, 1), 0);  // 16 bits per entry
+        fmask = filter.size() * 64 - 1;
     }
     uint64_t size() const { return key.size(); }
+    Cell& at(uint64_t s) { return cell[s % shards][s / shards]; }
+    const Cell& at(uint64_t s) const { return cell[s % shards][s / shards]; }
     int64_t find(uint64_t k) const {
-        for (uint64_t h = fmix(k) & imask;; h = (h + 1) & imask) {
-            const uint32_t s = islot[h];
-            if (s == 0) return -1;
-            if (ikey[h] == k) return (int64_t)s - 1;
+        const uint64_t h = fmix(k);
+        const uint64_t b = (h >> 20) & fmask;
+        if (!((filter[b >> 6] >> (b & 63)) & 1)) return -1;
+        for (uint64_t i = h & imask;; i = (i + 1) & imask) {
+            const Ent& e = index[i];
+            if (e.slot == 0) return -1;
+            if (e.key == k) return (int64_t)e.slot - 1;
         }
     }
     // Adds k (absent) with the given state; returns its slot.
     uint64_t add(uint64_t k, bool p, uint64_t v) {
         const uint64_t s = key.size();
         key.push_back(k);
-        present.push_back(p ? 1 : 0);
-        val.push_back(v);
-        uint64_t h = fmix(k) & imask;
-        while (islot[h]) h = (h + 1) & imask;
-        ikey[h] = k;
-        islot[h] = (uint32_t)(s + 1);
+        cell[s % shards].push_back(Cell{v, p ? 1u : 0u});
+        const uint64_t h = fmix(k);
+        const uint64_t b = (h >> 20) & fmask;
+        filter[b >> 6] |= 1ull << (b & 63);
+        uint64_t i = h & imask;
+        while (index[i].slot) i = (i + 1) & imask;
+        index[i] = Ent{k, s + 1};
         return s;
     }
 };
@@ -184,9 +206,11 @@ struct lslab_kvcg {
     uint8_t *h_ops = nullptr, *d_ops = nullptr;
     char *h_keys = nullptr, *h_vals = nullptr, *h_out = nullptr;
     char *d_keys = nullptr, *d_vals = nullptr, *d_out = nullptr;
-    // per chunk: cold op indices; per (chunk, shard): hot ops as slot << 32 | index
+    // per chunk: cold op indices; per (chunk, shard): hot ops as slot << 32 |
+    // index, and their results in the same order
     std::vector<std::vector<uint32_t>> cold;
     std::vector<std::vector<std::vector<uint64_t>>> hotl;
+    std::vector<std::vector<std::vector<Cell>>> hres;
     std::vector<uint64_t> off;
     lslab_kvcg_stats st{};
 };
@@ -263,9 +287,10 @@ int write_back(lslab_kvcg* kv, const std::vector<uint64_t>& slots, hipStream_t s
         const uint64_t m = std::min<uint64_t>(kv->max_batch, slots.size() - lo);
         for (uint64_t j = 0; j < m; ++j) {
             const uint64_t sl = slots[lo + j];
-            kv->h_ops[j] = kv->hot.present[sl] ? LSLAB_OP_PUT : LSLAB_OP_REMOVE;
+            const Cell& c = kv->hot.at(sl);
+            kv->h_ops[j] = c.present ? LSLAB_OP_PUT : LSLAB_OP_REMOVE;
             stw(kv->h_keys, j, kv->kb, kv->hot.key[sl]);
-            if (kv->is_map) stw(kv->h_vals, j, kv->vb, kv->hot.val[sl]);
+            if (kv->is_map) stw(kv->h_vals, j, kv->vb, c.val);
         }
         int rc = run_staged(kv, m, s);
         if (!rc) rc = finish(s);
@@ -312,29 +337,30 @@ int run_slice(lslab_kvcg* kv, const uint8_t* ops, const void* keys, const void* 
             }
         });
     }
-    // 3. cold ops on the device, hot ops on the workers meanwhile
+    // 3. cold ops on the device, hot ops on the workers meanwhile (results
+    // into per-shard lists: ops of one output cache line may be in any shard)
     int rc = run_staged(kv, ncold, s);
     if (nhot) {
         HotTier& H = kv->hot;
         const bool is_map = kv->is_map != 0;
         W.run([&](int sh) {
+            std::vector<Cell>& mine = H.cell[sh];
             for (int c = 0; c < T; ++c) {
+                std::vector<Cell>& res = kv->hres[c][sh];
+                res.clear();
                 for (uint64_t e : kv->hotl[c][sh]) {
                     const uint64_t sl = e >> 32, i = e & 0xffffffffull;
+                    Cell& st = mine[sl / (uint64_t)T];
                     const uint8_t op = opcode(ops, i);
-                    const bool was = H.present[sl] != 0;
-                    const uint64_t old = was ? H.val[sl] : 0;
-                    if (op == LSLAB_OP_GET) {
-                        write_out(kv, out, i, was, old);
-                    } else if (op == LSLAB_OP_PUT) {
-                        // a set insert reports true for present and new keys alike
-                        write_out(kv, out, i, is_map ? was : true, old);
-                        H.present[sl] = 1;
-                        H.val[sl] = is_map ? ldw(vals, i, vb) : 0;
-                    } else {
-                        write_out(kv, out, i, was, old);
-                        H.present[sl] = 0;
-                        H.val[sl] = 0;
+                    const uint64_t old = st.present ? st.val : 0;
+                    // a set insert reports true for present and new keys alike
+                    res.push_back(Cell{old, (op == LSLAB_OP_PUT && !is_map) ? 1u : st.present});
+                    if (op == LSLAB_OP_PUT) {
+                        st.present = 1;
+                        st.val = is_map ? ldw(vals, i, vb) : 0;
+                    } else if (op == LSLAB_OP_REMOVE) {
+                        st.present = 0;
+                        st.val = 0;
                     }
                 }
             }
@@ -343,16 +369,22 @@ int run_slice(lslab_kvcg* kv, const uint8_t* ops, const void* keys, const void* 
     const int frc = finish(s);
     if (rc) return rc;
     if (frc) return frc;
-    // 4. scatter the cold results
-    if (ncold) {
-        W.run([&](int c) {
-            uint64_t p = kv->off[c];
-            for (uint32_t i : kv->cold[c]) {
-                const auto r = staged_result(kv, p++);
-                write_out(kv, out, i, r.first, r.second);
+    // 4. every chunk writes the results of its own ops
+    W.run([&](int c) {
+        uint64_t p = kv->off[c];
+        for (uint32_t i : kv->cold[c]) {
+            const auto r = staged_result(kv, p++);
+            write_out(kv, out, i, r.first, r.second);
+        }
+        if (nhot) {
+            for (int sh = 0; sh < T; ++sh) {
+                const auto& hl = kv->hotl[c][sh];
+                const auto& hr = kv->hres[c][sh];
+                for (size_t k = 0; k < hl.size(); ++k)
+                    write_out(kv, out, hl[k] & 0xffffffffull, hr[k].present != 0, hr[k].val);
             }
-        });
-    }
+        }
+    });
     kv->st.hot_ops += nhot;
     kv->st.cold_ops += ncold;
     return LSLAB_OK;
@@ -381,7 +413,7 @@ extern "C" int lslab_kvcg_create(lslab_kvcg** out, lslab_table* t, uint64_t hot_
     kv->hot_cap = hot_capacity;
     kv->max_batch = max_batch;
     kv->ostride = is_map ? 2u * (uint32_t)vb : 1u;
-    kv->hot.reset(hot_capacity);
+    kv->hot.reset(hot_capacity, threads);
     int prev = 0;
     if (hipGetDevice(&prev) != hipSuccess || hipSetDevice(dev) != hipSuccess) {
         delete kv;
@@ -410,6 +442,7 @@ extern "C" int lslab_kvcg_create(lslab_kvcg** out, lslab_table* t, uint64_t hot_
     }
     kv->cold.resize(threads);
     kv->hotl.assign(threads, std::vector<std::vector<uint64_t>>(threads));
+    kv->hres.assign(threads, std::vector<std::vector<Cell>>(threads));
     kv->off.assign(threads, 0);
     kv->st.hot_capacity = hot_capacity;
     *out = kv;
@@ -434,7 +467,7 @@ extern "C" int lslab_kvcg_set_hot(lslab_kvcg* kv, const void* keys, uint64_t n,
                                   hipStream_t stream) {
     if (!kv || (n && !keys)) return LSLAB_ERR_INVALID;
     HotTier next;
-    next.reset(kv->hot_cap);
+    next.reset(kv->hot_cap, kv->workers->size());
     std::vector<uint64_t> promote;  // slots of `next` whose state the table holds
     for (uint64_t i = 0; i < n; ++i) {
         const uint64_t k = ldw(keys, i, kv->kb);
@@ -442,7 +475,8 @@ extern "C" int lslab_kvcg_set_hot(lslab_kvcg* kv, const void* keys, uint64_t n,
         if (next.size() == kv->hot_cap) return LSLAB_ERR_INVALID;
         const int64_t o = kv->hot.find(k);
         if (o >= 0) {
-            next.add(k, kv->hot.present[o] != 0, kv->hot.val[o]);
+            const Cell& c = kv->hot.at((uint64_t)o);
+            next.add(k, c.present != 0, c.val);
         } else {
             promote.push_back(next.add(k, false, 0));
         }
@@ -466,8 +500,7 @@ extern "C" int lslab_kvcg_set_hot(lslab_kvcg* kv, const void* keys, uint64_t n,
         if (rc) return rc;
         for (uint64_t j = 0; j < m; ++j) {
             const auto r = staged_result(kv, j);
-            next.present[promote[lo + j]] = r.first ? 1 : 0;
-            next.val[promote[lo + j]] = r.second;
+            next.at(promote[lo + j]) = Cell{r.second, r.first ? 1u : 0u};
         }
     }
     kv->hot = std::move(next);

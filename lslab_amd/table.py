@@ -351,7 +351,15 @@ class Kvcg:
         self._h = h
 
     def _host(self, a, dt):
-        a = self._np.ascontiguousarray(self._np.asarray(a).astype(self._np.uint64).astype(dt))
+        """a as a contiguous array of dt bit patterns (no copy when it already is one)."""
+        np = self._np
+        a = np.asarray(a)
+        if a.dtype != dt:
+            if a.dtype.kind in "iu" and a.dtype.itemsize == np.dtype(dt).itemsize:
+                a = a.view(dt)
+            else:
+                a = a.astype(np.uint64).astype(dt)
+        a = np.ascontiguousarray(a)
         return a, ctypes.c_void_p(a.ctypes.data)
 
     def set_hot(self, keys, stream=None):
@@ -364,8 +372,10 @@ class Kvcg:
         _lib.check(_lib.load().lslab_kvcg_sample_hot(self._h, sp, len(s), k, _stream(stream)),
                    "kvcg_sample_hot")
 
-    def mixed(self, ops, keys, values=None, stream=None):
-        """Returns (found u8[n], value u64[n]) as numpy arrays (value 0 for sets)."""
+    def mixed(self, ops, keys, values=None, out=None, stream=None, raw=False):
+        """Returns (found u8[n], value u64[n]) as numpy arrays (value 0 for sets),
+        or with raw=True the result buffer itself (out, if given: (n, 2) of the
+        value width for maps, n bytes for sets)."""
         np = self._np
         n = len(keys)
         o = np.ascontiguousarray(np.asarray(ops, dtype=np.uint8))
@@ -373,14 +383,21 @@ class Kvcg:
             raise ValueError("ops and keys differ in length")
         k, kp = self._host(keys, self._kdt)
         if self.is_map:
-            v, vp = self._host(values if values is not None else np.zeros(n), self._vdt)
-            out = np.zeros((n, 2), dtype=self._vdt)
+            v, vp = self._host(values if values is not None else np.zeros(n, np.uint64),
+                               self._vdt)
+            shape, odt = (n, 2), self._vdt
         else:
             vp = None
-            out = np.zeros(n, dtype=np.uint8)
+            shape, odt = (n,), np.uint8
+        if out is None:
+            out = np.empty(shape, dtype=odt)
+        elif out.shape != shape or out.dtype != odt or not out.flags.c_contiguous:
+            raise ValueError(f"out must be a contiguous {odt.__name__} array of shape {shape}")
         _lib.check(_lib.load().lslab_kvcg_mixed(self._h, ctypes.c_void_p(o.ctypes.data), kp, vp,
                                                 ctypes.c_void_p(out.ctypes.data), n,
                                                 _stream(stream)), "kvcg_mixed")
+        if raw:
+            return out
         if self.is_map:
             found = out.view(np.uint8).reshape(n, 2 * self.vb)[:, 0].copy()
             return found, out[:, 1].astype(np.uint64)
