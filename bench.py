@@ -358,6 +358,57 @@ def host_input(lslab_amd, torch, m, tun, ops, keys, vals, nb, hot_keys=1 << 16):
     return res
 
 
+def wide_keys(lslab_amd, torch, nb=8):
+    """Wide-key map, 128-B keys -> 8-B values (the legacy 128-B-key tests'
+    shape): 2^18 buckets prefilled with 2^22 keys (16 per bucket), then
+    batches of 2^20 ops over the keys, uniform: get-only calls (lock-free) and
+    95/5 get/put mixed calls (bucket locks).  Algorithmic bytes per get: the
+    128-B key in, its bucket's 128-B tag line, one 128-B key-arena entry, the
+    8-B value and a 9-B result = 401 B."""
+    nkeys, n_log2 = 1 << 22, 18
+    w = lslab_amd.Wide(128, 8, n_log2, 1 << 16)
+    mult = (torch.arange(16, dtype=torch.int64, device="cuda") * 2 + 1) * 0x9E3779B1
+
+    def rows(ids):
+        k = ids[:, None] * mult[None, :]
+        k[:, 0] = ids
+        return k.view(torch.uint8).reshape(ids.numel(), 128)
+
+    for lo in range(0, nkeys, BATCH):
+        ids = torch.arange(lo, lo + BATCH, dtype=torch.int64, device="cuda")
+        w.put(rows(ids), ids.view(torch.uint8).reshape(BATCH, 8))
+    g = torch.Generator(device="cuda")
+    g.manual_seed(0x3D1)
+    batches = []
+    for b in range(nb):
+        ids = torch.randint(0, nkeys, (BATCH,), device="cuda", generator=g)
+        ops = (torch.rand(BATCH, device="cuda", generator=g) < P_PUT).to(torch.uint8)
+        batches.append((rows(ids), ops, ids.view(torch.uint8).reshape(BATCH, 8)))
+    torch.cuda.synchronize()
+    w.get(batches[0][0])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k, _, _ in batches:
+        w.get(k)
+    torch.cuda.synchronize()
+    get_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for k, o, v in batches:
+        w.mixed(o, k, v)
+    torch.cuda.synchronize()
+    mixed_s = time.perf_counter() - t0
+    st = w.status()
+    if st:
+        raise RuntimeError(f"wide status {st}")
+    n = nb * BATCH
+    res = {"workload": "wide map 128-B keys -> 8-B values, 2^18 buckets, 2^22 keys, "
+                       "batches of 2^20 uniform ops",
+           "get_mops": round(n / get_s / 1e6, 2), "mixed_95_5_mops": round(n / mixed_s / 1e6, 2),
+           "bytes_per_get": 401, "get_gbs": round(401 * n / get_s / 1e9, 1)}
+    w.close()
+    return res
+
+
 def cpu_baseline(ops, keys, vals, nb_sample):
     """The oracle's sequential replay of nb_sample batches after the same prefill."""
     import numpy as np
@@ -443,6 +494,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-host-input", action="store_true")
+    ap.add_argument("--no-wide", action="store_true")
     ap.add_argument("--served-requests", type=int, default=20000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 1 GiB device copy first, as the HBM byte-counter reference")
@@ -555,6 +607,10 @@ def main():
         res["host_input"] = host_input(lslab_amd, torch, m, th, ops[W * BATCH:],
                                        keys[W * BATCH:], vals[W * BATCH:], min(K, 8))
         log(res["host_input"])
+    if not args.no_wide:
+        log("wide keys ...")
+        res["wide"] = wide_keys(lslab_amd, torch)
+        log(res["wide"])
     if not args.no_batched:
         log("batched form (4096 uneven batches) ...")
         res["batched"] = batched(lslab_amd, wl, torch, m, tuned["batched"])

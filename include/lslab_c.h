@@ -235,6 +235,40 @@ int lslab_chain_histogram(lslab_table* t, uint64_t* hist, unsigned nbins, hipStr
 int lslab_dump(lslab_table* t, void* keys_out, void* vals_out, uint64_t cap, uint64_t* count,
                hipStream_t stream);
 
+/* Wide-key tables (new): map / set with keys of 4..128 bytes and values of
+ * 0 (set) .. 128 bytes, both multiples of 4 -- the reference's legacy tests
+ * of 128-B keys (test/slabunified_test.cu.old:344-553: Key -> int*, Key ->
+ * Key).  A slab is one 128-B line of 30 key tags (32-bit hash
+ * fingerprints), the next link and the valid bitmap; keys and values live in
+ * arenas parallel to the slots, so a probe reads the tag line and confirms a
+ * tag match against the key arena.  Same op semantics, bucket locks and slab
+ * policy as the plain engine of the narrow tables (insert into the last
+ * slab's lowest empty slot; pool slabs never freed).  hash: 0 or
+ * LSLAB_HASH_IDENTITY (bucket = the key's first 8 bytes) or LSLAB_HASH_MIX.
+ * Inputs and outputs are DEVICE arrays: keys n * key_bytes, values n *
+ * value_bytes, out_found n bytes, out values n * value_bytes (either output
+ * may be NULL).  get reads without locks; put / remove / mixed lock their
+ * buckets.  A set reports insert as true, like lslab_set_insert.
+ * stats: slab_bytes counts the tag line plus the 30 slots' arena bytes. */
+typedef struct lslab_wide lslab_wide;
+int lslab_wide_create(lslab_wide** out, int key_bytes, int value_bytes, unsigned n_log2,
+                      uint64_t pool_slabs, int hash);
+int lslab_wide_destroy(lslab_wide* w);
+int lslab_wide_put(lslab_wide* w, const void* keys, const void* values, uint8_t* out_found,
+                   void* out_old, size_t n, hipStream_t stream);
+int lslab_wide_get(lslab_wide* w, const void* keys, uint8_t* out_found, void* out_values,
+                   size_t n, hipStream_t stream);
+int lslab_wide_remove(lslab_wide* w, const void* keys, uint8_t* out_found, void* out_old,
+                      size_t n, hipStream_t stream);
+int lslab_wide_mixed(lslab_wide* w, const uint8_t* ops, const void* keys, const void* values,
+                     uint8_t* out_found, void* out_values, size_t n, hipStream_t stream);
+int lslab_wide_status(lslab_wide* w, hipStream_t stream);
+int lslab_wide_stats(lslab_wide* w, lslab_stats* out, hipStream_t stream);
+int lslab_wide_dump(lslab_wide* w, void* keys_out, void* vals_out, uint64_t cap, uint64_t* count,
+                    hipStream_t stream);
+/* Slabs (heads + pool) of a wide table; 0 if the shape is invalid. */
+uint64_t lslab_wide_slab_count(unsigned n_log2, uint64_t pool_slabs);
+
 /* Shape of a table (new): 1 for a map, key and value widths in bytes (0 for
  * a set's value), and the HIP device it lives on. */
 int lslab_table_info(const lslab_table* t, int* is_map, int* key_bytes, int* value_bytes,

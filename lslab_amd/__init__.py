@@ -9,7 +9,7 @@ from ._lib import (COMBINE_BLOCK, COMBINE_WAVE, ENGINE_PLAIN, ENGINE_TWO_PHASE, 
                    HASH_IDENTITY, HASH_MIX, OP_GET, OP_PUT, OP_REMOVE, RECLAIM_OFF,
                    RECLAIM_ON, LslabError)
 
-__all__ = ["Map", "Set", "Kvcg", "tuning", "slab_bytes", "workload", "ENGINE_PLAIN", "ENGINE_TWO_PHASE",
+__all__ = ["Map", "Set", "Kvcg", "Wide", "tuning", "slab_bytes", "workload", "ENGINE_PLAIN", "ENGINE_TWO_PHASE",
            "COMBINE_WAVE", "COMBINE_BLOCK", "HASH_IDENTITY", "HASH_MIX", "RECLAIM_OFF",
            "RECLAIM_ON", "OP_GET", "OP_PUT", "OP_REMOVE", "LslabError"]
 
@@ -18,7 +18,7 @@ def __getattr__(name):
     # torch-dependent parts load lazily so that the C ABI can be inspected
     # without importing torch.
     import importlib
-    if name in ("Map", "Set", "Kvcg", "tuning", "slab_bytes"):
+    if name in ("Map", "Set", "Kvcg", "Wide", "tuning", "slab_bytes"):
         return getattr(importlib.import_module(__name__ + ".table"), name)
     if name in ("workload", "table", "autotune", "build"):
         return importlib.import_module(__name__ + "." + name)

@@ -44,6 +44,9 @@ EXPORTS = (
     "lslab_chain_histogram", "lslab_slab_bytes", "lslab_map_create_at", "lslab_set_create_at",
     "lslab_table_info", "lslab_kvcg_create", "lslab_kvcg_destroy", "lslab_kvcg_set_hot",
     "lslab_kvcg_sample_hot", "lslab_kvcg_mixed", "lslab_kvcg_flush", "lslab_kvcg_stats_get",
+    "lslab_wide_create", "lslab_wide_destroy", "lslab_wide_put", "lslab_wide_get",
+    "lslab_wide_remove", "lslab_wide_mixed", "lslab_wide_status", "lslab_wide_stats",
+    "lslab_wide_dump", "lslab_wide_slab_count",
 )
 
 SERVE_REJECTED = 0xFFFFFFFF
@@ -158,6 +161,15 @@ def load():
         "lslab_kvcg_mixed": [vp, vp, vp, vp, vp, u64, vp],
         "lslab_kvcg_flush": [vp, vp],
         "lslab_kvcg_stats_get": [vp, ctypes.POINTER(KvcgStats)],
+        "lslab_wide_create": [pp, i32, i32, ctypes.c_uint, u64, i32],
+        "lslab_wide_destroy": [vp],
+        "lslab_wide_put": [vp, vp, vp, vp, vp, sz, vp],
+        "lslab_wide_get": [vp, vp, vp, vp, sz, vp],
+        "lslab_wide_remove": [vp, vp, vp, vp, sz, vp],
+        "lslab_wide_mixed": [vp, vp, vp, vp, vp, vp, sz, vp],
+        "lslab_wide_status": [vp, vp],
+        "lslab_wide_stats": [vp, ctypes.POINTER(Stats), vp],
+        "lslab_wide_dump": [vp, vp, vp, u64, ctypes.POINTER(ctypes.c_uint64), vp],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -165,6 +177,8 @@ def load():
         f.restype = ctypes.c_int
     lib.lslab_slab_bytes.argtypes = [i32, i32, i32, ctypes.c_uint, u64]
     lib.lslab_slab_bytes.restype = u64
+    lib.lslab_wide_slab_count.argtypes = [ctypes.c_uint, u64]
+    lib.lslab_wide_slab_count.restype = u64
     lib.lslab_error_string.argtypes = [ctypes.c_int]
     lib.lslab_error_string.restype = ctypes.c_char_p
     _LIB = lib

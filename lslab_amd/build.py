@@ -2,8 +2,9 @@
 
 The kernels of the six table layouts are separate translation units
 (csrc/launch.hip compiled once per layout), compiled in parallel with the C
-ABI (csrc/lslab.hip), the workload generators and the host-side KVCG split
-(csrc/kvcg.cpp), then linked into one shared library.
+ABI (csrc/lslab.hip), the workload generators, the host-side KVCG split
+(csrc/kvcg.cpp) and the wide-key tables (csrc/wide.hip), then linked into
+one shared library.
 
 python -m lslab_amd.build [--verbose]
 """
@@ -17,7 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LAYOUTS = ["L_M3232", "L_M3264", "L_M6432", "L_M6464", "L_S32", "L_S64"]
 UNITS = ([("lslab", "csrc/lslab.hip", []), ("workload", "csrc/workload.hip", []),
-          ("kvcg", "csrc/kvcg.cpp", [])] +
+          ("kvcg", "csrc/kvcg.cpp", []), ("wide", "csrc/wide.hip", [])] +
          [(f"launch_{l.lower()}", "csrc/launch.hip", [f"-DLSLAB_LAYOUT={l}"]) for l in LAYOUTS])
 HEADERS = ["csrc/slab.hpp", "csrc/engine.hpp", "csrc/kernels.hpp"]
 OUT = os.path.join(HERE, "liblslab.so")

@@ -421,3 +421,96 @@ class Kvcg:
             self.close()
         except Exception:
             pass
+
+
+class Wide:
+    """Wide-key map / set (lslab_wide_*): keys of 4..128 bytes, values of 0 (set)
+    .. 128 bytes, both multiples of 4.  Keys and values are CUDA uint8 tensors
+    of shape (n, key_bytes) and (n, value_bytes); results are (found uint8[n],
+    values uint8[n, value_bytes])."""
+
+    def __init__(self, key_bytes, value_bytes=0, n_log2=10, pool_slabs=10000, hash=0):
+        self.kb, self.vb, self.n_log2 = key_bytes, value_bytes, n_log2
+        self.is_map = value_bytes > 0
+        h = ctypes.c_void_p()
+        _lib.check(_lib.load().lslab_wide_create(ctypes.byref(h), key_bytes, value_bytes, n_log2,
+                                                 pool_slabs, hash), "wide_create")
+        self._h = h
+
+    def _rows(self, t, width, what):
+        if not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype != torch.uint8:
+            raise TypeError(f"{what} must be a CUDA uint8 tensor")
+        if t.dim() != 2 or t.shape[1] != width or not t.is_contiguous():
+            raise ValueError(f"{what} must be a contiguous (n, {width}) tensor")
+        return t
+
+    def _outs(self, n):
+        found = torch.empty(n, dtype=torch.uint8, device="cuda")
+        vals = torch.empty((n, self.vb), dtype=torch.uint8, device="cuda") if self.vb else None
+        return found, vals
+
+    def _call(self, name, keys, values=None, ops=None, stream=None):
+        n = keys.shape[0]
+        self._rows(keys, self.kb, "keys")
+        if values is not None:
+            self._rows(values, self.vb, "values")
+        found, vals = self._outs(n)
+        lib = _lib.load()
+        if name == "mixed":
+            _ops(ops, n)
+            rc = lib.lslab_wide_mixed(self._h, _ptr(ops), _ptr(keys), _ptr(values), _ptr(found),
+                                      _ptr(vals), n, _stream(stream))
+        elif name == "put":
+            rc = lib.lslab_wide_put(self._h, _ptr(keys), _ptr(values), _ptr(found), _ptr(vals), n,
+                                    _stream(stream))
+        else:
+            f = lib.lslab_wide_get if name == "get" else lib.lslab_wide_remove
+            rc = f(self._h, _ptr(keys), _ptr(found), _ptr(vals), n, _stream(stream))
+        _lib.check(rc, "wide_" + name)
+        return found, vals
+
+    def put(self, keys, values=None, stream=None):
+        """Map put (out = previous value) / set insert (found always 1)."""
+        return self._call("put", keys, values, stream=stream)
+
+    def get(self, keys, stream=None):
+        return self._call("get", keys, stream=stream)
+
+    def remove(self, keys, stream=None):
+        return self._call("remove", keys, stream=stream)
+
+    def mixed(self, ops, keys, values=None, stream=None):
+        if self.is_map and values is None:
+            values = torch.zeros((keys.shape[0], self.vb), dtype=torch.uint8, device="cuda")
+        return self._call("mixed", keys, values, ops=ops, stream=stream)
+
+    def status(self, stream=None):
+        return _lib.load().lslab_wide_status(self._h, _stream(stream))
+
+    def stats(self, stream=None):
+        s = _lib.Stats()
+        _lib.check(_lib.load().lslab_wide_stats(self._h, ctypes.byref(s), _stream(stream)),
+                   "wide_stats")
+        return {f: getattr(s, f) for f, _ in _lib.Stats._fields_}
+
+    def dump(self, stream=None):
+        """(keys (m, key_bytes), values (m, value_bytes) or None), unordered."""
+        cap = max(self.stats(stream)["present"], 1)
+        k = torch.empty((cap, self.kb), dtype=torch.uint8, device="cuda")
+        v = torch.empty((cap, self.vb), dtype=torch.uint8, device="cuda") if self.vb else None
+        c = ctypes.c_uint64()
+        _lib.check(_lib.load().lslab_wide_dump(self._h, _ptr(k), _ptr(v), cap, ctypes.byref(c),
+                                               _stream(stream)), "wide_dump")
+        m = int(c.value)
+        return k[:m], (v[:m] if v is not None else None)
+
+    def close(self):
+        if self._h:
+            _lib.check(_lib.load().lslab_wide_destroy(self._h), "wide_destroy")
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
