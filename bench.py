@@ -484,7 +484,7 @@ def _cpu_model():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=32)
+    ap.add_argument("--steps", type=int, default=64)
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--lanes", type=int, default=0, help="override the tuned lanes per op")
     ap.add_argument("--combine", type=int, default=0,
