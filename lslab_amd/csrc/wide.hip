@@ -14,11 +14,11 @@
 // empty slot, no reuse: detail/traverse.h:54-55,98-124,
 // detail/device_allocator.h:62-73).
 //
-// Execution.  One op per 32-lane group (two per wavefront): lane l holds key
-// word l and value word l.  A probe loads the slab line (one word per lane),
-// ballots the lanes whose tag matches a valid slot, and confirms each
-// candidate by comparing the key arena entry word by word, again one word per
-// lane (one coalesced read per candidate).  Mutations and gets of mixed calls
+// Execution.  One op per 8-lane group (eight per wavefront): lane l holds
+// words l, l+8, l+16, l+24 of the key and of the value.  A probe loads the
+// slab line (four words per lane), ORs together the slots whose tag matches,
+// and confirms each candidate by comparing the key arena entry four words per
+// lane (coalesced reads, with the value read beside it).  Mutations and gets of mixed calls
 // hold the bucket's reader/writer lock (the reference protocol); read-only
 // calls take no locks.
 //
@@ -40,7 +40,8 @@ constexpr int WNEXT = 30;
 constexpr int WVALID = 31;
 constexpr uint32_t WMASK = (1u << WSLOTS) - 1u;
 constexpr uint64_t WSEED = 0x9E3779B97F4A7C15ull;
-constexpr int WG = 32;  // lanes per op
+constexpr int WG = 8;    // lanes per op (eight ops per wavefront)
+constexpr int WPL = 4;   // words of a 32-word line / key / value per lane
 
 struct WideDev {
     uint32_t* slabs;  // (nbuckets + pool) * 32 words
@@ -74,7 +75,13 @@ struct WideStats {
 __device__ __forceinline__ int wlane() { return (int)(__lane_id() & (WG - 1)); }
 // This group's bits of a wave ballot.
 __device__ __forceinline__ uint32_t wballot(bool p) {
-    return (uint32_t)(__ballot(p) >> (__lane_id() & WG));
+    return (uint32_t)(__ballot(p) >> (__lane_id() & ~(uint32_t)(WG - 1))) & ((1u << WG) - 1u);
+}
+// OR of v over the group's lanes.
+__device__ __forceinline__ uint32_t grp_or(uint32_t v) {
+#pragma unroll
+    for (int d = WG / 2; d >= 1; d >>= 1) v |= __shfl_xor(v, d, WG);
+    return v;
 }
 
 template <bool COH>
@@ -83,15 +90,34 @@ __device__ __forceinline__ uint32_t wld(const uint32_t* p) {
     return *p;
 }
 
-// Finds the key (kw words, this lane's word `kword`) in bucket b's chain.
-// On return slot >= 0: the key sits in slab s, slot `slot`, and `vfound` is
-// this lane's word of its value (loaded with the key words, so a hit costs no
-// third dependent load); slot < 0: absent, `last` is the chain's last slab and
-// `lastvalid` its valid bitmap.
+// A group holds a record of up to 32 words strided over its lanes: lane l
+// holds words l, l + 8, l + 16, l + 24, so each of the four accesses covers 32
+// contiguous bytes (one sector) -- also for agent-scope loads and
+// write-through stores, which are issued one word per lane.
+__device__ __forceinline__ int word_of(int k) { return WG * k + wlane(); }
+
+// This lane's words of a `cnt`-word record at p (zeros past cnt).
+template <bool COH>
+__device__ __forceinline__ void ld_words(const uint32_t* p, int cnt, uint32_t (&w)[WPL]) {
+#pragma unroll
+    for (int k = 0; k < WPL; ++k) w[k] = word_of(k) < cnt ? wld<COH>(p + word_of(k)) : 0u;
+}
+__device__ __forceinline__ void st_words(uint32_t* p, int cnt, const uint32_t (&w)[WPL]) {
+#pragma unroll
+    for (int k = 0; k < WPL; ++k)
+        if (word_of(k) < cnt) st_coh(p + word_of(k), w[k]);
+}
+
+// Finds the key (this lane's words `kw_` of it) in bucket b's chain.  On
+// return slot >= 0: the key sits in slab s, slot `slot`, and `vfound` holds
+// this lane's words of its value (loaded with the key words, so a hit costs
+// no third dependent load); slot < 0: absent, `last` is the chain's last slab
+// and `lastvalid` its valid bitmap.
 template <bool COH>
 __device__ __forceinline__ void wide_find(const WideDev& t, uint64_t b, uint32_t tag,
-                                          uint32_t kword, uint64_t& s, int& slot, uint64_t& last,
-                                          uint32_t& lastvalid, uint32_t& vfound, bool& ok) {
+                                          const uint32_t (&kw_)[WPL], uint64_t& s, int& slot,
+                                          uint64_t& last, uint32_t& lastvalid,
+                                          uint32_t (&vfound)[WPL], bool& ok) {
     const int gl = wlane();
     s = b;
     slot = -1;
@@ -100,19 +126,31 @@ __device__ __forceinline__ void wide_find(const WideDev& t, uint64_t b, uint32_t
     ok = true;
     const uint64_t nslabs = t.nbuckets + t.pool;
     for (uint64_t steps = 0;; ++steps) {
-        const uint32_t w = wld<COH>(t.slabs + s * 32 + gl);
-        const uint32_t valid = __shfl(w, WVALID, WG);
-        const uint32_t next = __shfl(w, WNEXT, WG);
-        uint32_t cand = wballot(gl < WSLOTS && ((valid >> gl) & 1u) && w == tag);
+        uint32_t w[WPL];
+        ld_words<COH>(t.slabs + s * 32, 32, w);  // lanes 6 and 7 hold next (word 30), valid (31)
+        const uint32_t next = __shfl(w[3], WNEXT - WG * 3, WG);
+        const uint32_t valid = __shfl(w[3], WVALID - WG * 3, WG);
+        uint32_t mine = 0;
+#pragma unroll
+        for (int k = 0; k < WPL; ++k) {
+            const int sl = word_of(k);
+            if (sl < WSLOTS && ((valid >> sl) & 1u) && w[k] == tag) mine |= 1u << sl;
+        }
+        uint32_t cand = grp_or(mine);
         while (cand) {
             const int j = __builtin_ctz(cand);
             cand &= cand - 1u;
             const uint64_t e = s * WSLOTS + (uint64_t)j;
-            const uint32_t kk = gl < t.kw ? wld<COH>(t.keys + e * (uint64_t)t.kw + gl) : 0u;
-            const uint32_t vv = gl < t.vw ? wld<COH>(t.vals + e * (uint64_t)t.vw + gl) : 0u;
-            if (wballot(gl < t.kw && kk != kword) == 0u) {
+            uint32_t kk[WPL], vv[WPL];
+            ld_words<COH>(t.keys + e * (uint64_t)t.kw, t.kw, kk);
+            ld_words<COH>(t.vals + e * (uint64_t)t.vw, t.vw, vv);
+            bool diff = false;
+#pragma unroll
+            for (int k = 0; k < WPL; ++k) diff |= kk[k] != kw_[k];
+            if (wballot(diff) == 0u) {
                 slot = j;
-                vfound = vv;
+#pragma unroll
+                for (int k = 0; k < WPL; ++k) vfound[k] = vv[k];
                 break;
             }
         }
@@ -132,26 +170,27 @@ __device__ __forceinline__ void wide_find(const WideDev& t, uint64_t b, uint32_t
 }
 
 // One op under its bucket's lock (COH: agent-coherent accesses) or, for a
-// read-only call, without it.  Results: found, this lane's word of the old
+// read-only call, without it.  Results: found, this lane's words of the old
 // value, and ok = false when the op could not complete (pool exhausted,
 // damaged chain).
 template <bool COH>
 __device__ __forceinline__ void wide_apply(const WideDev& t, uint8_t op, uint64_t b, uint32_t tag,
-                                           uint32_t kword, uint32_t vword, bool& found,
-                                           uint32_t& old, bool& ok) {
+                                           const uint32_t (&kw_)[WPL],
+                                           const uint32_t (&vw_)[WPL], bool& found,
+                                           uint32_t (&old)[WPL], bool& ok) {
     const int gl = wlane();
     uint64_t s, last;
     int slot;
-    uint32_t lastvalid, vfound = 0;
-    wide_find<COH>(t, b, tag, kword, s, slot, last, lastvalid, vfound, ok);
+    uint32_t lastvalid;
+    uint32_t vfound[WPL] = {0u, 0u, 0u, 0u};
+    wide_find<COH>(t, b, tag, kw_, s, slot, last, lastvalid, vfound, ok);
     found = slot >= 0;
-    old = found ? vfound : 0u;
+#pragma unroll
+    for (int k = 0; k < WPL; ++k) old[k] = found ? vfound[k] : 0u;
     if (!ok) return;
-    uint32_t* vp = nullptr;
-    if (found && gl < t.vw) vp = t.vals + (s * WSLOTS + (uint64_t)slot) * (uint64_t)t.vw + gl;
     if (op == OP_PUT) {
         if (found) {
-            if (gl < t.vw) st_coh(vp, vword);
+            st_words(t.vals + (s * WSLOTS + (uint64_t)slot) * (uint64_t)t.vw, t.vw, vw_);
         } else {
             const uint32_t empty = ~lastvalid & WMASK;
             uint64_t ts = last;
@@ -176,8 +215,8 @@ __device__ __forceinline__ void wide_apply(const WideDev& t, uint8_t op, uint64_
                 fresh = true;
             }
             const uint64_t e = ts * WSLOTS + (uint64_t)tj;
-            if (gl < t.kw) st_coh(t.keys + e * (uint64_t)t.kw + gl, kword);
-            if (gl < t.vw) st_coh(t.vals + e * (uint64_t)t.vw + gl, vword);
+            st_words(t.keys + e * (uint64_t)t.kw, t.kw, kw_);
+            st_words(t.vals + e * (uint64_t)t.vw, t.vw, vw_);
             if (gl == 0) st_coh(t.slabs + ts * 32 + tj, tag);
             if (fresh) {
                 if (gl == 0) st_coh(t.slabs + ts * 32 + WVALID, 1u);
@@ -197,7 +236,8 @@ __device__ __forceinline__ void wide_apply(const WideDev& t, uint8_t op, uint64_
     }
 }
 
-// One 32-lane group per op.  LOCKED: every op holds its bucket's lock (shared
+// One 8-lane group per op (lane l holds words l, l+8, l+16, l+24 of the key,
+// the value and a probed slab line).  LOCKED: every op holds its bucket's lock (shared
 // for gets); otherwise (read-only calls) no locks.
 template <bool LOCKED>
 __global__ __launch_bounds__(256) void k_wide(WideDev t, WideOps o, uint64_t n) {
@@ -209,10 +249,15 @@ __global__ __launch_bounds__(256) void k_wide(WideDev t, WideOps o, uint64_t n) 
         op = o.ops ? o.ops[i] : o.uop;
         if (op > OP_REMOVE) op = OP_GET;
     }
-    const uint32_t kword = (act && gl < t.kw) ? o.keys[i * (uint64_t)t.kw + gl] : 0u;
-    const uint32_t vword =
-        (act && op == OP_PUT && o.vals && gl < t.vw) ? o.vals[i * (uint64_t)t.vw + gl] : 0u;
-    uint64_t m = gl < t.kw ? mix64((((uint64_t)gl << 32) | kword) ^ WSEED) : 0ull;
+    uint32_t kw_[WPL] = {0u, 0u, 0u, 0u}, vw_[WPL] = {0u, 0u, 0u, 0u};
+    if (act) ld_words<false>(o.keys + i * (uint64_t)t.kw, t.kw, kw_);
+    if (act && op == OP_PUT && o.vals) ld_words<false>(o.vals + i * (uint64_t)t.vw, t.vw, vw_);
+    uint64_t m = 0;
+#pragma unroll
+    for (int k = 0; k < WPL; ++k) {
+        const int w = word_of(k);
+        if (w < t.kw) m ^= mix64((((uint64_t)w << 32) | kw_[k]) ^ WSEED);
+    }
 #pragma unroll
     for (int d = WG / 2; d >= 1; d >>= 1) {
         const uint32_t lo = __shfl_xor((uint32_t)m, d, WG);
@@ -221,14 +266,14 @@ __global__ __launch_bounds__(256) void k_wide(WideDev t, WideOps o, uint64_t n) 
     }
     const uint64_t h = mix64(m);
     const uint32_t tag = (uint32_t)h;
-    const uint32_t w0 = __shfl(kword, 0, WG);
-    const uint32_t w1 = t.kw > 1 ? __shfl(kword, 1, WG) : 0u;
+    const uint32_t w0 = __shfl(kw_[0], 0, WG);
+    const uint32_t w1 = __shfl(kw_[0], 1, WG);  // 0 when the key is one word
     const uint64_t b = (t.hash_mix ? (h >> 32) : ((uint64_t)w1 << 32 | w0)) & t.mask;
 
     bool found = false, ok = false;
-    uint32_t old = 0;
+    uint32_t old[WPL] = {0u, 0u, 0u, 0u};
     if constexpr (!LOCKED) {
-        if (act) wide_apply<false>(t, op, b, tag, kword, vword, found, old, ok);
+        if (act) wide_apply<false>(t, op, b, tag, kw_, vw_, found, old, ok);
     } else {
         const bool shared = op == OP_GET;
         uint32_t* lw = t.locks + b;
@@ -244,7 +289,7 @@ __global__ __launch_bounds__(256) void k_wide(WideDev t, WideOps o, uint64_t n) 
                 if (gl == 0) got = shared ? try_lock_shared(lw) : try_lock_excl(lw);
                 got = __shfl(got, 0, WG);
                 if (got) {
-                    wide_apply<true>(t, op, b, tag, kword, vword, found, old, ok);
+                    wide_apply<true>(t, op, b, tag, kw_, vw_, found, old, ok);
                     wait_vm();
                     if (gl == 0) {
                         if (shared) unlock_shared(lw);
@@ -271,7 +316,11 @@ __global__ __launch_bounds__(256) void k_wide(WideDev t, WideOps o, uint64_t n) 
     // a set insert reports true for present and new keys alike (not when it failed)
     const bool rf = (!t.is_map && op == OP_PUT) ? ok : found;
     if (o.found && gl == 0) o.found[i] = rf ? 1 : 0;
-    if (o.out_vals && gl < t.vw) o.out_vals[i * (uint64_t)t.vw + gl] = found ? old : 0u;
+    if (o.out_vals) {
+#pragma unroll
+        for (int k = 0; k < WPL; ++k)
+            if (word_of(k) < t.vw) o.out_vals[i * (uint64_t)t.vw + word_of(k)] = old[k];
+    }
 }
 
 // Present keys and longest chain: one thread per bucket.
@@ -291,7 +340,7 @@ __global__ __launch_bounds__(256) void k_wide_stats(WideDev t, WideStats* st) {
     atomicMax(&st->max_chain, (unsigned long long)len);
 }
 
-// Copies every present key (and value) out: one 32-lane group per slab.
+// Copies every present key (and value) out: one 8-lane group per slab.
 __global__ __launch_bounds__(256) void k_wide_dump(WideDev t, uint64_t nslabs, uint32_t* keys,
                                                    uint32_t* vals, uint64_t cap,
                                                    unsigned long long* counter) {
@@ -307,8 +356,12 @@ __global__ __launch_bounds__(256) void k_wide_dump(WideDev t, uint64_t nslabs, u
         idx = grp_bcast((uint64_t)idx, 0, WG);
         if (idx >= cap) continue;
         const uint64_t e = s * WSLOTS + (uint64_t)j;
-        if (keys && gl < t.kw) keys[idx * t.kw + gl] = t.keys[e * t.kw + gl];
-        if (vals && gl < t.vw) vals[idx * t.vw + gl] = t.vals[e * t.vw + gl];
+#pragma unroll
+        for (int k = 0; k < WPL; ++k) {
+            const int w = word_of(k);
+            if (keys && w < t.kw) keys[idx * t.kw + w] = t.keys[e * t.kw + w];
+            if (vals && w < t.vw) vals[idx * t.vw + w] = t.vals[e * t.vw + w];
+        }
     }
 }
 
