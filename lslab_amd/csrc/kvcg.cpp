@@ -340,6 +340,10 @@ int run_slice(lslab_kvcg* kv, const uint8_t* ops, const void* keys, const void* 
     // 3. cold ops on the device, hot ops on the workers meanwhile (results
     // into per-shard lists: ops of one output cache line may be in any shard)
     int rc = run_staged(kv, ncold, s);
+    if (rc) {
+        (void)finish(s);  // the hot tier stays as it was
+        return rc;
+    }
     if (nhot) {
         HotTier& H = kv->hot;
         const bool is_map = kv->is_map != 0;
@@ -367,7 +371,6 @@ int run_slice(lslab_kvcg* kv, const uint8_t* ops, const void* keys, const void* 
         });
     }
     const int frc = finish(s);
-    if (rc) return rc;
     if (frc) return frc;
     // 4. every chunk writes the results of its own ops
     W.run([&](int c) {

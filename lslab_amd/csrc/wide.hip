@@ -400,9 +400,13 @@ bool capturing(hipStream_t s) {
     return hipStreamIsCapturing(s, &st) == hipSuccess && st != hipStreamCaptureStatusNone;
 }
 
+// Key and value records are read and written as 32-bit words.
+bool aligned4(const void* p) { return ((uintptr_t)p & 3u) == 0; }
+
 int run(lslab_wide* w, const WideOps& o, uint64_t n, bool locked, hipStream_t s) {
     if (n == 0) return LSLAB_OK;
-    if (n > (1ull << 31)) return LSLAB_ERR_INVALID;
+    if (n > (1ull << 31) || !aligned4(o.keys) || !aligned4(o.vals) || !aligned4(o.out_vals))
+        return LSLAB_ERR_INVALID;
     if (!capturing(s)) WCHK(hipStreamWaitEvent(s, w->order_ev, 0));
     const uint64_t blocks = (n * WG + 255) / 256;
     if (locked)

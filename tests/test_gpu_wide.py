@@ -188,3 +188,16 @@ def test_wide_conflicts_serial_order(lslab, oracle_mod, vb):
                                      pf, pvv)
         assert bad == 0, (bi, first)
     assert w.status() == 0
+
+
+def test_wide_rejects_misaligned_records(lslab, oracle_mod):
+    """Key and value records are read as 32-bit words: a buffer that is not
+    4-byte aligned is refused before any device work."""
+    say("wide_misaligned")
+    w = lslab.Wide(16, 8, 4, 100)
+    raw = torch.zeros(10 * 16 + 1, dtype=torch.uint8, device="cuda")
+    with pytest.raises(lslab.LslabError):
+        w.get(raw[1:].view(10, 16))
+    f, _ = w.get(raw[:160].view(10, 16))
+    assert int(f.sum()) == 0
+    assert w.status() == 0
