@@ -143,28 +143,23 @@ __device__ __forceinline__ bool grid_sync(BarState* bar, uint32_t nblocks, bool 
     return ok_s != 0;
 }
 
-// Batches of at most SOLO_MAX ops run on block 0 alone, with block barriers
-// between their phases instead of grid barriers, which saves the two grid
-// barriers of a small batch.  Their final-mutation candidates all fit in the
-// block's LDS list.  Measured on the 4096-batch stream (sizes 32..65536): 256
-// gains ~2%; 1024 loses ~10%, because one block's dependent-latency chain
-// over several rounds of ops then costs more than the barriers.
-#ifndef LSLAB_SOLO_MAX
-#define LSLAB_SOLO_MAX 256
-#endif
-constexpr uint64_t SOLO_MAX = LSLAB_SOLO_MAX;
-static_assert(SOLO_MAX <= REP_LDS, "a solo batch keeps its candidates in LDS");
-
-// Persistent batched kernel: batches in order, each as two-phase sub-batches
-// separated by grid barriers.  A run of consecutive small batches goes to
-// block 0 alone (see SOLO_MAX) while the other blocks wait at the grid barrier
-// that opens the next large batch.
+// Persistent batched kernel: batches in order.  Consecutive batches that
+// together fit one sub-batch form a run: phase R of each batch in turn, with
+// a grid barrier after each, then one phase W for the whole run.  Mutations of
+// the run chain through one combining table, so a later batch's mutations
+// follow the earlier batches' in their keys' chains, and a later batch's gets
+// read the state the earlier batches left (run_state).  A run saves, per batch
+// after its first, one grid barrier and a phase W.  A batch larger than a
+// sub-batch runs alone, as sub-batches with a phase W each.  A run is also
+// capped so that no group handles more ops than a sub-batch gives it, which
+// bounds every block's list of final-mutation candidates by its fixed region.
 template <class L, int G, bool BC, bool RC>
 __global__ __launch_bounds__(PERSIST_THREADS) void k_batches(TableDev t, OpsDev o, OutDev out,
                                                  const uint64_t* offsets, uint64_t nbatches,
                                                  uint64_t submax, BarState* bar) {
     const uint64_t ng = (uint64_t)gridDim.x * blockDim.x / G;
     const uint64_t gid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+    const uint64_t per_group = (submax + ng - 1) / ng;  // ops a group's list share holds
     uint32_t* rc = rep_counter();
     RepEnt* rb = t.rep + block_rep_base_fixed(submax, ng, blockDim.x / G, blockIdx.x);
     if (threadIdx.x == 0) *rc = 0;
@@ -172,53 +167,51 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_batches(TableDev t, OpsDev 
         comb_init<PERSIST_THREADS>();
     else
         __syncthreads();
-    bool solo_run = false;  // block 0 has run small batches since the last grid barrier
-    for (uint64_t b = 0; b < nbatches; ++b) {
-        const uint64_t lo = offsets[b], hi = offsets[b + 1];
-        if (hi > lo && hi - lo <= SOLO_MAX && hi - lo <= submax) {
-            if (blockIdx.x == 0) {
-                phase_r_range<L, G, PERSIST_THREADS, BC>(t, o, out, lo, hi - lo,
-                                                         threadIdx.x / G, blockDim.x / G, rb,
+    auto fail = [&]() {
+        if (threadIdx.x == 0)
+            __hip_atomic_fetch_or(t.status, ST_TIMEOUT, __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT);
+    };
+    uint64_t b = 0;
+    while (b < nbatches) {
+        const uint64_t lo = offsets[b];
+        // the run [b, e): every block computes the same one
+        uint64_t e = b, load = 0;
+        while (e < nbatches) {
+            const uint64_t mm = offsets[e + 1] - offsets[e];
+            if (mm > submax) break;
+            const uint64_t c = (mm + ng - 1) / ng;
+            if (e > b && (load + c > per_group || offsets[e + 1] - lo > submax)) break;
+            load += c;
+            ++e;
+        }
+        if (e == b) {  // a batch larger than a sub-batch
+            const uint64_t hi = offsets[b + 1];
+            for (uint64_t slo = lo; slo < hi; slo += submax) {
+                const uint64_t m = min(submax, hi - slo);
+                phase_r_range<L, G, PERSIST_THREADS, BC>(t, o, out, slo, m, gid, ng, rb,
                                                          rep_lds(), REP_LDS);
-                wait_vm();
-                __syncthreads();
-                phase_w_block<L, G, RC>(t, o, lo, rb, *rc, rep_lds(), REP_LDS);
-                wait_vm();
-                __syncthreads();
-                if (threadIdx.x == 0) *rc = 0;
-                __syncthreads();
+                if (!grid_sync(bar, gridDim.x, false)) return fail();
+                phase_w_block<L, G, RC>(t, o, slo, rb, *rc, rep_lds(), REP_LDS);
+                if (!grid_sync(bar, gridDim.x, true, rc)) return fail();
             }
-            solo_run = true;
+            b = e + 1;
             continue;
         }
-        if (solo_run) {
-            // the other blocks see block 0's writes before the large batch
-            if (!grid_sync(bar, gridDim.x, true)) {
-                if (threadIdx.x == 0)
-                    __hip_atomic_fetch_or(t.status, ST_TIMEOUT, __ATOMIC_RELAXED,
-                                          __HIP_MEMORY_SCOPE_AGENT);
-                return;
-            }
-            solo_run = false;
+        for (uint64_t x = b; x < e; ++x) {
+            const uint64_t first = offsets[x] - lo, end = offsets[x + 1] - lo;
+            if (end == first) continue;  // empty batch (block-uniform)
+            if (x == b)  // first == 0
+                phase_r_range<L, G, PERSIST_THREADS, BC, false>(t, o, out, lo, end, gid, ng, rb,
+                                                                rep_lds(), REP_LDS);
+            else
+                phase_r_range<L, G, PERSIST_THREADS, BC, true>(t, o, out, lo, end, gid, ng, rb,
+                                                               rep_lds(), REP_LDS, first);
+            if (!grid_sync(bar, gridDim.x, false)) return fail();
         }
-        for (uint64_t slo = lo; slo < hi; slo += submax) {
-            const uint64_t m = min(submax, hi - slo);
-            phase_r_range<L, G, PERSIST_THREADS, BC>(t, o, out, slo, m, gid, ng, rb, rep_lds(),
-                                                     REP_LDS);
-            if (!grid_sync(bar, gridDim.x, false)) {
-                if (threadIdx.x == 0)
-                    __hip_atomic_fetch_or(t.status, ST_TIMEOUT, __ATOMIC_RELAXED,
-                                          __HIP_MEMORY_SCOPE_AGENT);
-                return;
-            }
-            phase_w_block<L, G, RC>(t, o, slo, rb, *rc, rep_lds(), REP_LDS);
-            if (!grid_sync(bar, gridDim.x, true, rc)) {
-                if (threadIdx.x == 0)
-                    __hip_atomic_fetch_or(t.status, ST_TIMEOUT, __ATOMIC_RELAXED,
-                                          __HIP_MEMORY_SCOPE_AGENT);
-                return;
-            }
-        }
+        phase_w_block<L, G, RC>(t, o, lo, rb, *rc, rep_lds(), REP_LDS);
+        if (!grid_sync(bar, gridDim.x, true, rc)) return fail();
+        b = e;
     }
 }
 
