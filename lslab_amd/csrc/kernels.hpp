@@ -333,18 +333,21 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_serve(TableDev t, OpsDev o,
                 phase_r_range<L, G, PERSIST_THREADS, BC>(t, o, out, slo, m, gid, ng, rb, rep_lds(),
                                                      REP_LDS);
                 if (!grid_sync(bar, gridDim.x, false)) goto fail;
+                // Every result of the tick is written in phase R: its requests
+                // complete here.  Phase W only brings the table up to date for
+                // the next tick.
+                if (slo + m >= hi && blockIdx.x == 0) {
+                    if (threadIdx.x == 0) stamp_s = now_ticks() - ss->t0;
+                    __syncthreads();
+                    for (uint64_t r = r_lo + threadIdx.x; r < r_hi; r += blockDim.x) {
+                        sv.complete[r] = stamp_s;
+                        sv.tick_of[r] = (uint32_t)tick;
+                    }
+                    __syncthreads();
+                    if (threadIdx.x == 0) ss->ticks = tick + 1;
+                }
                 phase_w_block<L, G, RC>(t, o, slo, rb, *rc, rep_lds(), REP_LDS);
                 if (!grid_sync(bar, gridDim.x, true, rc)) goto fail;
-            }
-            if (blockIdx.x == 0) {
-                if (threadIdx.x == 0) stamp_s = now_ticks() - ss->t0;
-                __syncthreads();
-                for (uint64_t r = r_lo + threadIdx.x; r < r_hi; r += blockDim.x) {
-                    sv.complete[r] = stamp_s;
-                    sv.tick_of[r] = (uint32_t)tick;
-                }
-                __syncthreads();
-                if (threadIdx.x == 0) ss->ticks = tick + 1;
             }
         }
     }
