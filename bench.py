@@ -501,7 +501,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(every box here has one GPU; DESIGN.md describes how the table would shard)",
+              "(a multi-GPU table would shard by bucket; this benchmark runs one GPU's share)",
               file=sys.stderr)
         sys.exit(2)
     import torch

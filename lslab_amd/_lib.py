@@ -124,8 +124,7 @@ def load():
     if _LIB is not None:
         return _LIB
     if not os.path.exists(LIB_PATH):
-        raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() or "
-                           "python -m lslab_amd.build")
+        raise RuntimeError(f"{LIB_PATH} is not built; run python -m lslab_amd.build")
     lib = ctypes.CDLL(LIB_PATH)
     vp, sz, u64, i32 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint64, ctypes.c_int
     pp = ctypes.POINTER(ctypes.c_void_p)
