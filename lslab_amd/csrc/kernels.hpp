@@ -153,6 +153,14 @@ __device__ __forceinline__ bool grid_sync(BarState* bar, uint32_t nblocks, bool 
 // sub-batch runs alone, as sub-batches with a phase W each.  A run is also
 // capped so that no group handles more ops than a sub-batch gives it, which
 // bounds every block's list of final-mutation candidates by its fixed region.
+// Only batches of at most RUN_JOIN_MAX ops join a run after its first: every
+// get of a later batch reads its key's combining-table head with an
+// agent-scope load, and under skew a hot key's head is read by all of the
+// batch's gets of that key while the batch's own mutations exchange on it
+// (two 2^20-op Zipf 0.99 batches in one run ran at a third of their speed
+// alone).
+constexpr uint64_t RUN_JOIN_MAX = 1ull << 16;
+
 template <class L, int G, bool BC, bool RC>
 __global__ __launch_bounds__(PERSIST_THREADS) void k_batches(TableDev t, OpsDev o, OutDev out,
                                                  const uint64_t* offsets, uint64_t nbatches,
@@ -181,7 +189,9 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_batches(TableDev t, OpsDev 
             const uint64_t mm = offsets[e + 1] - offsets[e];
             if (mm > submax) break;
             const uint64_t c = (mm + ng - 1) / ng;
-            if (e > b && (load + c > per_group || offsets[e + 1] - lo > submax)) break;
+            if (e > b && (mm > RUN_JOIN_MAX || load + c > per_group ||
+                          offsets[e + 1] - lo > submax))
+                break;
             load += c;
             ++e;
         }
