@@ -564,3 +564,50 @@ def test_caller_slab_memory(lslab, oracle_mod, is_map, kb, vb, engine):
     arena.fill_(1)
     torch.cuda.synchronize()
     assert int(arena.sum()) == need + 512
+
+
+@pytest.mark.parametrize("is_map", [True, False])
+def test_exec_batches_run_boundaries(lslab, oracle_mod, is_map):
+    """Batches around the run limits: a batch of exactly 2^16 ops may join a run,
+    one op more starts its own; a batch larger than the sub-batch runs alone
+    in sub-batches; empty batches inside runs.  Each batch is checked in order
+    against the oracle's serial-order checker, then the final contents."""
+    say("exec_batches_run_boundaries", is_map)
+    rng = np.random.default_rng(41)
+    kb, vb = 8, 8
+    tbl = make(lslab, is_map, kb, vb, 8, 1 << 15, 1, 4, subbatch=1 << 18, combine=1)
+    ref = Replay(is_map, kb, vb, 8, pool_slabs=1 << 15)
+    sizes = np.array([100, 1 << 16, 50, 0, (1 << 16) + 1, 7, 300000, 5, 1 << 16, 1 << 16,
+                      1 << 16, 1 << 16, 3], np.int64)
+    n = int(sizes.sum())
+    space = 40000
+    p = 1.0 / np.arange(1, space + 1) ** 0.99
+    p /= p.sum()
+    if is_map:
+        ranks = rng.choice(space, size=n, p=p)
+    else:
+        ranks = np.concatenate([rng.permutation(400000)[:s] for s in sizes])
+    keys = ranks.astype(np.uint64) * np.uint64(0x9E3779B97F4A7C15)
+    ops = rng.choice([0, 1, 2], size=n, p=[0.7, 0.25, 0.05]).astype(np.uint8)
+    vals = rng.integers(1, 2**63, size=n, dtype=np.uint64)
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    d_offs = torch.from_numpy(offs).cuda()
+    d_ops = torch.from_numpy(ops).cuda()
+    if is_map:
+        f, v = tbl.exec_batches(d_offs, d_ops, to_dev(keys, kb), to_dev(vals, vb))
+        f, v = f.cpu().numpy(), to_host(v.contiguous(), vb)
+    else:
+        f = tbl.exec_batches(d_offs, d_ops, to_dev(keys, kb)).cpu().numpy()
+        v = np.zeros(n, np.uint64)
+    assert tbl.status() == 0
+    for b in range(len(sizes)):
+        lo, hi = offs[b], offs[b + 1]
+        if hi == lo:
+            continue
+        bad, first = ref.check_batch(ops[lo:hi], keys[lo:hi], vals[lo:hi], f[lo:hi], v[lo:hi])
+        assert bad == 0, (b, first)
+    if is_map:
+        k, vv = dump_sorted(tbl, is_map, kb, vb)
+        rk, rv = oracle_dump_sorted(ref)
+        np.testing.assert_array_equal(k, rk)
+        np.testing.assert_array_equal(vv, rv)
