@@ -4,6 +4,8 @@
 //                              allocator is a caller-owned arena: two maps'
 //                              slab arrays carved from one hipMalloc)
 //   ContainsInsertRemoveTest   test/set_test.cu:65-106
+//   PutRemoveTest_128B         test/slabunified_test.cu.old:386-474 (128-B keys
+//   PutRemoveTest_128Bto128B   test/slabunified_test.cu.old:476-541  on lslab::wide_map)
 // GetPutTest also runs on a table created with lslab::mix_hash (the
 // reference's Hash template parameter with a non-identity functor).
 // Exit code 0 = all passed.
@@ -16,6 +18,7 @@
 
 #include <lslab/map.h>
 #include <lslab/set.h>
+#include <lslab/wide.h>
 
 #define ASSERT(x, y) \
     if (!(x)) throw std::runtime_error((y));
@@ -138,6 +141,46 @@ static void contains_insert_remove(int engine) {
     ASSERT(s.status() == LSLAB_OK, "device status");
 }
 
+// The legacy test's 128-byte Key: i in the first 8 bytes, zeros after
+// (test/slabunified_test.cu.old:344-364).
+struct Key128 {
+    uint64_t w[16];
+};
+static Key128 key128(uint64_t i) {
+    Key128 k{};
+    k.w[0] = i;
+    return k;
+}
+
+// 100 rounds: insert keys 1..1000 with fresh values (each reads back "absent"),
+// then remove them (each returns the value that round inserted).
+template <typename V, typename MakeV>
+static void put_remove_128b(MakeV make_value) {
+    constexpr int size = 1000;
+    lslab::wide_map<Key128, V> m(10, 10000);
+    std::vector<Key128> hk(size);
+    for (int j = 0; j < size; ++j) hk[j] = key128(j + 1);
+    dvec<Key128> keys(size);
+    dvec<V> vals(size), old(size);
+    dvec<bool> found(size);
+    keys.from(hk);
+    for (int rep = 0; rep < 100; ++rep) {
+        std::vector<V> hv(size);
+        for (int j = 0; j < size; ++j) hv[j] = make_value(rep, j);
+        vals.from(hv);
+        m.put(keys.p, vals.p, found.p, old.p, size);
+        HIPOK(hipDeviceSynchronize());
+        for (uint8_t b : found.bytes()) ASSERT(b == 0, "insert of an absent key found it");
+        for (uint8_t b : old.bytes()) ASSERT(b == 0, "insert of an absent key read a value");
+        m.remove(keys.p, found.p, old.p, size);
+        HIPOK(hipDeviceSynchronize());
+        for (uint8_t b : found.bytes()) ASSERT(b == 1, "remove missed a key");
+        ASSERT(old.bytes() == vals.bytes(), "remove returned another value");
+    }
+    ASSERT(m.status() == LSLAB_OK, "device status");
+    ASSERT(m.stats().present == 0, "keys left after the last round");
+}
+
 int main() {
     try {
         for (int engine = 0; engine < 2; ++engine) {
@@ -150,6 +193,10 @@ int main() {
             get_put<lslab::mix_hash<int>>(10000, engine);
             std::printf("GetPutTest mix_hash engine %d ok\n", engine);
         }
+        put_remove_128b<uint64_t>([](int rep, int j) { return ((uint64_t)rep << 32) | (j + 1); });
+        std::printf("PutRemoveTest_128B ok\n");
+        put_remove_128b<Key128>([](int rep, int) { return key128(rep); });
+        std::printf("PutRemoveTest_128Bto128B ok\n");
     } catch (const std::exception& e) {
         std::printf("FAILED: %s\n", e.what());
         return 1;

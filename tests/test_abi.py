@@ -56,7 +56,10 @@ def test_cpp_headers_compile():
     """The header-only C++ API compiles on its own (host-only translation unit)."""
     import subprocess
     import tempfile
-    src = ('#include <lslab/map.h>\n#include <lslab/set.h>\n'
+    src = ('#include <lslab/map.h>\n#include <lslab/set.h>\n#include <lslab/wide.h>\n'
+           'struct K20 { unsigned w[5]; };\n'
+           'template class lslab::wide_map<K20, unsigned long long>;\n'
+           'template class lslab::wide_set<K20>;\n'
            'int main(){ static_assert(sizeof(lslab::found_value<unsigned long long>)==16,"");'
            ' static_assert(sizeof(lslab::found_value<unsigned>)==8,"");'
            ' static_assert(sizeof(lslab::key_value<unsigned,unsigned long long>)==16,"");'
