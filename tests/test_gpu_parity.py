@@ -611,3 +611,43 @@ def test_exec_batches_run_boundaries(lslab, oracle_mod, is_map):
         rk, rv = oracle_dump_sorted(ref)
         np.testing.assert_array_equal(k, rk)
         np.testing.assert_array_equal(vv, rv)
+
+
+@pytest.mark.parametrize("combine", [1, 2])
+def test_exec_batches_reclaim_runs(lslab, oracle_mod, combine):
+    """The batched kernel on a reclaim table: many small batches (runs sharing
+    one phase W) that insert and remove heavily in few buckets, so pool slabs
+    are emptied, unlinked and reused inside runs.  Each batch follows a serial
+    order, the final contents are the replay's, and the pool slabs in use are
+    at least what the final contents need."""
+    say("exec_batches_reclaim_runs", combine)
+    rng = np.random.default_rng(1234 + combine)
+    tun = lslab.tuning(lanes_per_op=4, engine=1, max_subbatch=1 << 14, combine=combine,
+                       reclaim=lslab.RECLAIM_ON)
+    tbl = lslab.Map(8, 8, 3, 1 << 12, tun)
+    ref = Replay(True, 8, 8, 3, pool_slabs=1 << 12, reclaim=True)
+    sizes = np.exp(rng.uniform(np.log(1), np.log(3000), size=120)).astype(np.int64)
+    n = int(sizes.sum())
+    keys = rng.integers(1, 3000, size=n).astype(np.uint64)
+    ops = rng.choice([0, 1, 2], size=n, p=[0.3, 0.4, 0.3]).astype(np.uint8)
+    vals = rng.integers(1, 2**63, size=n, dtype=np.uint64)
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    f, v = tbl.exec_batches(torch.from_numpy(offs).cuda(), torch.from_numpy(ops).cuda(),
+                            to_dev(keys, 8), to_dev(vals, 8))
+    f, v = f.cpu().numpy(), to_host(v.contiguous(), 8)
+    assert tbl.status() == 0
+    for b in range(len(sizes)):
+        lo, hi = offs[b], offs[b + 1]
+        if hi == lo:
+            continue
+        bad, first = ref.check_batch(ops[lo:hi], keys[lo:hi], vals[lo:hi], f[lo:hi], v[lo:hi])
+        assert bad == 0, (b, first)
+    k, vv = dump_sorted(tbl, True, 8, 8)
+    rk, rv = oracle_dump_sorted(ref)
+    np.testing.assert_array_equal(k, rk)
+    np.testing.assert_array_equal(vv, rv)
+    st = tbl.stats()
+    assert st["present"] == len(rk)
+    per_bucket = np.bincount((rk & np.uint64(7)).astype(np.int64), minlength=8)
+    need = int(sum(max(0, -(-(int(c) - 31) // 31)) for c in per_bucket))
+    assert need <= st["slabs_used"] <= 1 << 12
