@@ -519,11 +519,21 @@ def main():
         if args.combine:
             v["combine"] = args.combine
     if args.copy_calibration:
+        # 1 GiB streaming copy: the byte counters' scale for streaming access
         src = torch.ones(1 << 27, dtype=torch.int64, device="cuda")
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+        # random 128-B rows: 2^22 of 2^24 rows of a 2 GiB tensor (512 MiB read at
+        # random, 32 MiB of indices, 512 MiB written): the scale for random lines
+        rows = torch.zeros((1 << 24, 16), dtype=torch.int64, device="cuda")
+        g = torch.Generator(device="cuda")
+        g.manual_seed(7)
+        idx = torch.randint(0, 1 << 24, (1 << 22,), device="cuda", generator=g)
+        picked = torch.index_select(rows, 0, idx)
+        torch.cuda.synchronize()
+        del rows, idx, picked
     th = tuned["headline"]
     m = lslab_amd.Map(8, 8, N_LOG2, 1 << 16,
                       lslab_amd.tuning(lanes_per_op=th["lanes_per_op"],
@@ -595,7 +605,7 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": (round(measured_traffic_per_op() * K * BATCH)
                         if measured_traffic_per_op() else None),
-            "traffic_source": "rocprofv3 FETCH_SIZE (x2 copy calibration) + WRITE_SIZE per op "
+            "traffic_source": "rocprofv3 FETCH_SIZE (x1.063, random-line calibration) + WRITE_SIZE per op "
                               "of k_batches, profiles/k_batches_map_u64_zipf099_headline.md, "
                               "times the ops of this launch",
             "kernel": "k_batches<Layout<u64,u64,map>, lanes_per_op>",
