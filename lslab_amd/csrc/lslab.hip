@@ -93,25 +93,43 @@ static uint64_t rep_len(const lslab_table* t) {
     return t->tun.max_subbatch + (uint64_t)t->num_cus * PERSIST_THREADS;
 }
 
+// Device memory the library holds for t (caller-provided slab memory excluded).
+static uint64_t table_device_bytes(const lslab_table* t) {
+    return (t->own_slabs ? (t->nbuckets + t->pool) * t->slab_bytes : 0) + t->nbuckets * 4 +
+           sizeof(DevState) + (t->gt_cap + 1) * 12 + rep_len(t) * sizeof(RepEnt) +
+           rep_cnt_len(t->submax) * 4;
+}
+
+// Sizes the two-phase scratch for t->tun.max_subbatch.  The new buffers are
+// allocated before the old ones are freed, so on failure the table keeps its
+// previous, consistent scratch.
 static int alloc_scratch(lslab_table* t) {
     uint64_t cap = 1;
     while (cap < 2 * t->tun.max_subbatch) cap <<= 1;
+    uint64_t* gt_key = nullptr;
+    uint32_t* gt_head = nullptr;
+    RepEnt* rep = nullptr;
+    uint32_t* rep_cnt = nullptr;
+    if (hipMalloc(&gt_key, (cap + 1) * sizeof(uint64_t)) != hipSuccess ||
+        hipMalloc(&gt_head, (cap + 1) * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&rep, rep_len(t) * sizeof(RepEnt)) != hipSuccess ||
+        hipMalloc(&rep_cnt, rep_cnt_len(t->tun.max_subbatch) * sizeof(uint32_t)) != hipSuccess ||
+        hipMemset(gt_key, 0xff, (cap + 1) * sizeof(uint64_t)) != hipSuccess ||
+        hipMemset(gt_head, 0, (cap + 1) * sizeof(uint32_t)) != hipSuccess) {
+        (void)hipFree(gt_key);
+        (void)hipFree(gt_head);
+        (void)hipFree(rep);
+        (void)hipFree(rep_cnt);
+        return LSLAB_ERR_OOM;
+    }
     (void)hipFree(t->dev.gt_key);
     (void)hipFree(t->dev.gt_head);
     (void)hipFree(t->dev.rep);
     (void)hipFree(t->dev.rep_cnt);
-    t->dev.gt_key = nullptr;
-    t->dev.gt_head = nullptr;
-    t->dev.rep = nullptr;
-    t->dev.rep_cnt = nullptr;
-    if (hipMalloc(&t->dev.gt_key, (cap + 1) * sizeof(uint64_t)) != hipSuccess ||
-        hipMalloc(&t->dev.gt_head, (cap + 1) * sizeof(uint32_t)) != hipSuccess ||
-        hipMalloc(&t->dev.rep, rep_len(t) * sizeof(RepEnt)) != hipSuccess ||
-        hipMalloc(&t->dev.rep_cnt, rep_cnt_len(t->tun.max_subbatch) * sizeof(uint32_t)) !=
-            hipSuccess)
-        return LSLAB_ERR_OOM;
-    HIPCHK(hipMemset(t->dev.gt_key, 0xff, (cap + 1) * sizeof(uint64_t)));
-    HIPCHK(hipMemset(t->dev.gt_head, 0, (cap + 1) * sizeof(uint32_t)));
+    t->dev.gt_key = gt_key;
+    t->dev.gt_head = gt_head;
+    t->dev.rep = rep;
+    t->dev.rep_cnt = rep_cnt;
     t->gt_cap = cap;
     t->submax = t->tun.max_subbatch;
     t->dev.gt_mask = (uint32_t)(cap - 1);
@@ -225,9 +243,7 @@ static int create_common(lslab_table** out, TableType type, bool is_map, int kb,
     t->dev.pool = pool;
     t->dev.mask = (uint32_t)(t->nbuckets - 1);
     t->dev.hash_mix = t->tun.hash == LSLAB_HASH_MIX;
-    t->device_bytes = (t->own_slabs ? slab_total : 0) + t->nbuckets * 4 + sizeof(DevState) +
-                      (t->gt_cap + 1) * 12 + rep_len(t) * sizeof(RepEnt) +
-                      rep_cnt_len(t->submax) * 4;
+    t->device_bytes = table_device_bytes(t);
     *out = t;
     return LSLAB_OK;
 }
@@ -298,11 +314,18 @@ extern "C" int lslab_set_tuning(lslab_table* t, const lslab_tuning* tuning) {
     if (tu.hash != t->tun.hash) return LSLAB_ERR_INVALID;  // the table's buckets depend on it
     if (tu.reclaim != t->tun.reclaim) return LSLAB_ERR_INVALID;  // fixed with the table
     const bool resize = tu.max_subbatch != t->tun.max_subbatch;
+    const lslab_tuning prev = t->tun;
     t->tun = tu;
     if (resize) {
         // The scratch may still be in use by queued calls of this table.
-        HIPCHK(hipDeviceSynchronize());
-        return alloc_scratch(t);
+        if (hipDeviceSynchronize() != hipSuccess) {
+            t->tun = prev;
+            return LSLAB_ERR_HIP;
+        }
+        const int rc = alloc_scratch(t);
+        if (rc) t->tun = prev;  // the previous scratch stays, sized for the previous tuning
+        else t->device_bytes = table_device_bytes(t);
+        return rc;
     }
     return LSLAB_OK;
 }

@@ -651,3 +651,27 @@ def test_exec_batches_reclaim_runs(lslab, oracle_mod, combine):
     per_bucket = np.bincount((rk & np.uint64(7)).astype(np.int64), minlength=8)
     need = int(sum(max(0, -(-(int(c) - 31) // 31)) for c in per_bucket))
     assert need <= st["slabs_used"] <= 1 << 12
+
+
+def test_set_tuning_resizes_scratch(lslab, oracle_mod):
+    """Changing max_subbatch re-sizes the two-phase scratch (device_bytes
+    follows it) and the table keeps its contents and results."""
+    say("set_tuning_resize")
+    tbl = make(lslab, True, 8, 8, 6, 1 << 10, 1, 4, subbatch=1 << 12)
+    ref = Replay(True, 8, 8, 6, pool_slabs=1 << 10)
+    rng = np.random.default_rng(8)
+    before = tbl.stats()["device_bytes"]
+    for sub in (1 << 16, 1 << 10, 1 << 12):
+        tbl.set_tuning(lslab.tuning(lanes_per_op=4, engine=1, max_subbatch=sub))
+        keys = rng.choice(np.arange(1, 20000, dtype=np.uint64), size=3000, replace=False)
+        ops = rng.choice([0, 1, 2], size=3000).astype(np.uint8)
+        vals = rng.integers(1, 2**63, size=3000, dtype=np.uint64)
+        f, v = run_batch(tbl, True, 8, 8, ops, keys, vals)
+        rf, rv, fail = ref.apply(ops, keys, vals)
+        assert fail == 0
+        np.testing.assert_array_equal(f, rf)
+        np.testing.assert_array_equal(v, rv)
+        if sub == 1 << 16:
+            assert tbl.stats()["device_bytes"] > before
+    assert tbl.stats()["device_bytes"] == before
+    assert tbl.status() == 0
