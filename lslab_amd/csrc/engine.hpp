@@ -844,6 +844,113 @@ __device__ __forceinline__ void compact_locked(const TableDev& t, typename L::K 
     run_op<L, G, true, true>(t, OP_COMPACT, key, typename L::V(0), rf, rv, ok);
 }
 
+// Valid bitmap and next link of slab s, agent-coherent (one 8-byte load for
+// u64 keys, where both sit in slot 0).
+template <class L>
+__device__ __forceinline__ void load_meta(const TableDev& t, uint64_t s, uint32_t& valid,
+                                          uint32_t& next) {
+    if constexpr (sizeof(typename L::K) == 8) {
+        const uint64_t w = ld_coh(reinterpret_cast<const uint64_t*>(valid_word<L>(t, s)));
+        valid = (uint32_t)w;
+        next = (uint32_t)(w >> 32);
+    } else {
+        valid = ld_coh(valid_word<L>(t, s));
+        next = ld_coh(next_word<L>(t, s));
+    }
+}
+
+// Phase W insert (reference placement policy) of a key that phase R found
+// absent.  No other op of the sub-batch writes this key and no phase-W op
+// compares keys, so the slot can be claimed before its key is written: the
+// group's lane 0 walks to the chain's last slab and claims its lowest empty
+// slot with an atomic OR of the valid word (a lost race retries with the
+// bitmap that OR returned).  Only a full last slab takes the bucket lock, to
+// link a fresh slab holding the key in its first slot, unless another group
+// linked one first.  Concurrent inserts into one bucket thus fill the same
+// slots, and use the same number of slabs, as the locked insert in some
+// serial order.  The lock is taken and released within one pass of the
+// wave-uniform retry loop (as in run_op).
+template <class L, int G>
+__device__ __forceinline__ void w_insert(const TableDev& t, typename L::K key, typename L::V val) {
+    const uint64_t b = bucket_of(t, (uint64_t)key);
+    const int gl = Grp<G>::lane();
+    uint64_t s = b, hops = 0, t0 = 0;
+    uint32_t backoff = 1;
+    int done = 0;
+    for (;;) {
+        int busy = 0;
+        if (!done && gl == 0) {
+            uint32_t valid, nx;
+            load_meta<L>(t, s, valid, nx);
+            while (nx != 0 && !done) {
+                if (nx < t.nbuckets || nx >= t.nbuckets + t.pool || ++hops > t.pool) {
+                    __hip_atomic_fetch_or(t.status, ST_CORRUPT, __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+                    done = 1;
+                } else {
+                    s = nx;
+                    load_meta<L>(t, s, valid, nx);
+                }
+            }
+            for (uint32_t empty = ~valid & L::CAPMASK; empty && !done;) {
+                const int slot = __builtin_ctz(empty);
+                const uint32_t old = __hip_atomic_fetch_or(valid_word<L>(t, s), 1u << slot,
+                                                           __ATOMIC_RELAXED,
+                                                           __HIP_MEMORY_SCOPE_AGENT);
+                if (old & (1u << slot)) {
+                    empty = ~old & L::CAPMASK;
+                } else {
+                    st_coh(key_slot<L>(t, s, slot), key);
+                    if constexpr (L::IS_MAP) st_coh(val_slot<L>(t, s, slot), val);
+                    done = 1;
+                }
+            }
+            if (!done) {
+                uint32_t* lw = t.locks + b;
+                if (try_lock_excl(lw)) {
+                    uint32_t v2, n2;
+                    load_meta<L>(t, s, v2, n2);
+                    if (n2 == 0 && (~v2 & L::CAPMASK) == 0) {
+                        const uint64_t idx = __hip_atomic_fetch_add(
+                            t.alloc_ctr, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        if (idx < t.pool) {
+                            // A pool slab is all zeros until it is linked.
+                            const uint64_t ns = t.nbuckets + idx;
+                            st_coh(key_slot<L>(t, ns, L::SLOT0), key);
+                            if constexpr (L::IS_MAP) st_coh(val_slot<L>(t, ns, L::SLOT0), val);
+                            st_coh(valid_word<L>(t, ns), 1u << L::SLOT0);
+                            wait_vm();
+                            st_coh(next_word<L>(t, s), (uint32_t)ns);
+                        } else {
+                            __hip_atomic_fetch_or(t.status, ST_POOL_EXHAUSTED, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT);
+                        }
+                        done = 1;
+                    }
+                    // else a slot was freed or a slab linked: next pass retries
+                    wait_vm();
+                    unlock_excl(lw);
+                } else {
+                    busy = 1;
+                    const uint64_t now = now_ticks();
+                    if (t0 == 0) t0 = now;
+                    if (now - t0 > SPIN_LIMIT_TICKS) {
+                        __hip_atomic_fetch_or(t.status, ST_TIMEOUT, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+                        done = 1;
+                    }
+                }
+            }
+        }
+        done = grp_bcast(done, 0, G);
+        if (__ballot(!done) == 0) break;
+        if (__ballot(busy)) {
+            for (uint32_t i = 0; i < backoff; ++i) __builtin_amdgcn_s_sleep(4);
+            backoff = backoff < 64 ? backoff * 2 : 64;
+        }
+    }
+}
+
 // Phase W of one final-mutation candidate: if it is still its key's last
 // mutation (nothing exchanged after it), it writes the key's final state and
 // frees the combining-table entry.
@@ -883,9 +990,13 @@ __device__ __forceinline__ void phase_w_ent(const TableDev& t, const OpsDev& o, 
             if (((old & ~(1u << slot)) & L::CAPMASK) == 0) compact_locked<L, G>(t, key);
         }
     } else if (op == OP_PUT) {
-        bool rf, ok;
-        V rv;
-        run_op<L, G, true, RC>(t, OP_PUT, key, val, rf, rv, ok);
+        if constexpr (RC) {
+            bool rf, ok;
+            V rv;
+            run_op<L, G, true, RC>(t, OP_PUT, key, val, rf, rv, ok);
+        } else {
+            w_insert<L, G>(t, key, val);
+        }
     }
 }
 
