@@ -409,6 +409,81 @@ def wide_keys(lslab_amd, torch, nb=8):
     return res
 
 
+def _timed(torch, fn):
+    """Runs fn() between two events on the current stream; returns seconds."""
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / 1e3
+
+
+def other_configs(lslab_amd, wl, torch):
+    """The two configurations that are parity cases rather than the headline,
+    timed with events: map<u32,u32> on 2^16 buckets (put 2^20 distinct keys in
+    one call, then 2^20 gets, half of them absent), and set<u64> on 2^24 buckets
+    (2^28 inserts in 16 calls, 2^26 contains with 80% present, then a remove of
+    half the keys in 16 calls).  Throughputs in millions of ops per second, each
+    call's answers checked against what the key generation implies.  Lanes per
+    op as measured best for each op on these shapes: 8 for plain-engine
+    mutations, 4 for two-phase puts, 2 for lookups."""
+    res = {}
+    n = 1 << 20
+    keys = wl.keys(n, 4, first=0, seed=0)
+    vals = wl.values(n, 4, first=0, seed=1)
+    absent = wl.keys(n // 2, 4, first=n, seed=0)
+    queries = torch.cat([keys[: n // 2], absent])
+    for engine, name, lanes in ((lslab_amd.ENGINE_PLAIN, "plain", 8),
+                                (lslab_amd.ENGINE_TWO_PHASE, "two_phase", 4)):
+        for _rep in range(2):  # the first table warms the launch path up; the second is timed
+            m = lslab_amd.Map(4, 4, 16, 1 << 14,
+                              lslab_amd.tuning(lanes_per_op=lanes, engine=engine, max_subbatch=n))
+            put_s = _timed(torch, lambda: m.put(keys, vals))
+            m.set_tuning(lslab_amd.tuning(lanes_per_op=2, engine=engine, max_subbatch=n))
+            got = {}
+            get_s = _timed(torch, lambda: got.setdefault("r", m.get(queries)))
+            found = got["r"][0]
+            if m.status() or int(found[: n // 2].sum()) != n // 2 or int(found[n // 2:].sum()):
+                raise RuntimeError("plain small config: wrong results")
+            m.close()
+        res[f"u32_map_put_{name}_mops"] = round(n / put_s / 1e6, 1)
+        res[f"u32_map_get_{name}_mops"] = round(n / get_s / 1e6, 1)
+    del keys, vals, absent, queries
+    mut, look = lslab_amd.tuning(lanes_per_op=8), lslab_amd.tuning(lanes_per_op=2)
+    s = lslab_amd.Set(8, 24, 1 << 16, mut)
+    total, chunk = 1 << 28, 1 << 24
+    ins_s = 0.0
+    for lo in range(0, total, chunk):
+        k = wl.keys(chunk, 8, first=lo, seed=0)
+        ins_s += _timed(torch, lambda: s.insert(k))
+        del k
+    q = 1 << 26
+    qk = torch.cat([wl.keys(q * 4 // 5, 8, first=0, seed=0),
+                    wl.keys(q - q * 4 // 5, 8, first=total, seed=0)])
+    s.set_tuning(look)
+    got = {}
+    con_s = _timed(torch, lambda: got.setdefault("r", s.contains(qk)))
+    if int(got["r"].sum()) != q * 4 // 5:
+        raise RuntimeError("set at scale: wrong contains results")
+    del qk, got
+    s.set_tuning(mut)
+    rm_s, removed = 0.0, 0
+    for lo in range(0, total, chunk):
+        k = wl.keys(chunk, 8, first=lo, seed=0)[::2].contiguous()  # half of the keys
+        rm_s += _timed(torch, lambda: s.remove(k))
+        removed += k.numel()
+        del k
+    if s.status() or s.stats()["present"] != total - removed:
+        raise RuntimeError("set at scale: wrong contents")
+    s.close()
+    res.update({"set_insert_mops": round(total / ins_s / 1e6, 1),
+                "set_contains_mops": round(q / con_s / 1e6, 1),
+                "set_remove_mops": round(removed / rm_s / 1e6, 1)})
+    return res
+
+
 def cpu_baseline(ops, keys, vals, nb_sample):
     """The oracle's sequential replay of nb_sample batches after the same prefill."""
     import numpy as np
@@ -495,6 +570,7 @@ def main():
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-host-input", action="store_true")
     ap.add_argument("--no-wide", action="store_true")
+    ap.add_argument("--no-configs", action="store_true")
     ap.add_argument("--served-requests", type=int, default=20000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 1 GiB device copy first, as the HBM byte-counter reference")
@@ -617,6 +693,10 @@ def main():
         res["host_input"] = host_input(lslab_amd, torch, m, th, ops[W * BATCH:],
                                        keys[W * BATCH:], vals[W * BATCH:], min(K, 8))
         log(res["host_input"])
+    if not args.no_configs:
+        log("plain small map and set at scale ...")
+        res["other_configs"] = other_configs(lslab_amd, wl, torch)
+        log(res["other_configs"])
     if not args.no_wide:
         log("wide keys ...")
         res["wide"] = wide_keys(lslab_amd, torch)

@@ -357,7 +357,7 @@ __device__ __forceinline__ void run_op(const TableDev& t, uint8_t op, typename L
         const int gl = Grp<G>::lane();
         const bool shared = (op == OP_GET);
         uint32_t* lw = t.locks + b;
-        bool done = false;
+        bool done = false, first = true;
         uint64_t t0 = 0;
         uint32_t backoff = 1;
         // The loop exits only when every lane of the wave is done (a
@@ -368,7 +368,9 @@ __device__ __forceinline__ void run_op(const TableDev& t, uint8_t op, typename L
         for (;;) {
             if (!done) {
                 int got = 0;
-                if (gl == 0) got = shared ? try_lock_shared(lw) : try_lock_excl(lw);
+                if (gl == 0)
+                    got = shared ? try_lock_shared(lw) : try_lock_excl(lw, first);
+                first = false;
                 got = __shfl(got, 0, G);
                 if (got) {
                     // Inside the lock every slab access is agent-coherent, so

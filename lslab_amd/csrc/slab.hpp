@@ -432,17 +432,22 @@ __device__ __forceinline__ uint64_t ld_agent(const uint64_t* p) {
 
 // Wave lock primitives on a bucket's 32-bit word (one lane calls them).
 // bit 31 = held exclusively, bit 30 = a writer is waiting, bits 0..29 = number
-// of readers.  Attempts read the word first and only then issue the RMW
-// (test-and-test-and-set), and a writer that fails raises the waiting bit,
+// of readers.  A writer's first attempt issues its compare-and-swap at once
+// (an uncontended lock costs one round trip, and a failed CAS changes
+// nothing); its retries, and every reader attempt, read the word first and
+// only then issue the RMW (test-and-test-and-set): a reader's RMW on a held
+// lock would have to be undone.  A writer that fails raises the waiting bit,
 // which turns new readers away until it gets in -- so a hot bucket's stream
 // of readers cannot starve its writers.
 constexpr uint32_t LOCK_WAITING = 0x40000000u;
 
-__device__ __forceinline__ bool try_lock_excl(uint32_t* w) {
-    uint32_t v = ld_coh(w);
+__device__ __forceinline__ bool try_lock_excl(uint32_t* w, bool first = false) {
+    uint32_t v = first ? 0u : ld_coh(w);
     if (v == 0 || v == LOCK_WAITING) {
-        return __hip_atomic_compare_exchange_strong(w, &v, LOCK_WRITER, __ATOMIC_RELAXED,
-                                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (__hip_atomic_compare_exchange_strong(w, &v, LOCK_WRITER, __ATOMIC_RELAXED,
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            return true;
+        if (!first) return false;
     }
     if (!(v & LOCK_WAITING))
         __hip_atomic_fetch_or(w, LOCK_WAITING, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

@@ -277,7 +277,7 @@ __global__ __launch_bounds__(256) void k_wide(WideDev t, WideOps o, uint64_t n) 
     } else {
         const bool shared = op == OP_GET;
         uint32_t* lw = t.locks + b;
-        bool done = !act;
+        bool done = !act, first = true;
         uint64_t t0 = 0;
         uint32_t backoff = 1;
         // Wave-uniform exit: with a divergent exit the compiler could move the
@@ -286,7 +286,9 @@ __global__ __launch_bounds__(256) void k_wide(WideDev t, WideOps o, uint64_t n) 
         for (;;) {
             if (!done) {
                 int got = 0;
-                if (gl == 0) got = shared ? try_lock_shared(lw) : try_lock_excl(lw);
+                if (gl == 0)
+                    got = shared ? try_lock_shared(lw) : try_lock_excl(lw, first);
+                first = false;
                 got = __shfl(got, 0, WG);
                 if (got) {
                     wide_apply<true>(t, op, b, tag, kw_, vw_, found, old, ok);

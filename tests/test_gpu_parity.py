@@ -232,6 +232,45 @@ def test_pool_exhaustion_status(lslab, oracle_mod):
     assert s.status() == 0  # cleared
 
 
+@pytest.mark.parametrize("lanes", [1, 4, 32])
+@pytest.mark.parametrize("is_map,kb,vb", [(True, 8, 8), (True, 4, 4), (False, 4, 0)])
+def test_two_phase_inserts_one_bucket_pair(lslab, oracle_mod, is_map, kb, vb, lanes):
+    """Thousands of distinct keys inserted into 2 buckets in one two-phase
+    sub-batch: phase W's inserts race for slots of the same last slab and for
+    linking the next one.  Slab count and chain length equal the sequential
+    replay's, every key reads back its value; with a pool too small, the table
+    stops at its exact capacity with POOL_EXHAUSTED."""
+    say("two_phase_inserts_one_bucket_pair", is_map, kb, lanes)
+    rng = np.random.default_rng(kb + lanes)
+    n = 6000
+    keys = np.unique(rng.integers(1, 2**31, size=n + 100, dtype=np.uint64))[:n]
+    rng.shuffle(keys)
+    vals = rng.integers(1, 2**31, size=n, dtype=np.uint64)
+    cap = 31 if (kb == 8 or is_map) else 30  # keys per slab
+    tbl = make(lslab, is_map, kb, vb, 1, 400, 1, lanes)
+    ref = Replay(is_map, kb, vb, 1, pool_slabs=400, slab_keys=cap)
+    ops = np.full(n, OP_PUT, np.uint8)
+    run_batch(tbl, is_map, kb, vb, ops, keys, vals)
+    ref.apply(ops, keys, vals if is_map else None)
+    assert tbl.status() == 0
+    st = tbl.stats()
+    assert (st["present"], st["slabs_used"], st["max_chain"]) == (n, ref.slabs_used,
+                                                                  ref.max_chain)
+    f, v = run_batch(tbl, is_map, kb, vb, np.zeros(n, np.uint8), keys, vals)
+    assert f.all()
+    if is_map:
+        np.testing.assert_array_equal(v, vals)
+    small = make(lslab, is_map, kb, vb, 1, 20, 1, lanes)
+    run_batch(small, is_map, kb, vb, ops, keys, vals)
+    assert small.status() == 4
+    st = small.stats()
+    assert (st["present"], st["slabs_used"]) == (cap * 22, 20)
+    f, v = run_batch(small, is_map, kb, vb, np.zeros(n, np.uint8), keys, vals)
+    assert int(f.sum()) == cap * 22
+    if is_map:
+        np.testing.assert_array_equal(v[f == 1], vals[f == 1])
+
+
 @pytest.mark.parametrize("engine", ENGINES)
 def test_plain_small_config(lslab, oracle_mod, engine):
     """2^20 distinct uint32 keys, 2^16 buckets, one put batch, then 2^20 gets
