@@ -149,7 +149,9 @@ int lslab_get_tuning(const lslab_table* t, lslab_tuning* out);
  *            (detail/traverse.h:88-91) but map.h does not expose it.
  *            out[i] = {was present, removed value}
  * mixed:     new; per-op opcodes (LSLAB_OP_*), out[i] = {found, value} with the
- *            meaning of the op's single-type call (put: {was present, old}). */
+ *            meaning of the op's single-type call (put: {was present, old}).
+ *            values holds n entries, one per op (ignored for non-puts but
+ *            readable), or is null when no op is a put. */
 int lslab_map_put_pairs(lslab_table* t, const void* pairs, void* out_old, size_t n,
                         hipStream_t stream);
 int lslab_map_put(lslab_table* t, const void* keys, const void* values, void* out_old, size_t n,

@@ -965,7 +965,12 @@ __device__ __forceinline__ void phase_w_ent(const TableDev& t, const OpsDev& o, 
     uint8_t op;
     K key;
     V val;
-    load_op<L>(o, lo + e.li, op, key, val);
+    // The value is loaded whatever the opcode (every op has a value slot), so
+    // the record's loads and the head check share one round trip.
+    const uint64_t i = lo + e.li;
+    op = ld_opcode(o, i);
+    key = ld_stride<K>(o.keys, o.kstride, i);
+    val = (L::IS_MAP && o.vals) ? ld_stride<V>(o.vals, o.vstride, i) : V(0);
     if (ld_agent(&t.gt_head[e.g]) != e.li + 1u) return;  // a later mutation of this key exists
     if (gl == 0) {
         __hip_atomic_store(&t.gt_head[e.g], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
