@@ -296,9 +296,13 @@ __device__ __forceinline__ void apply_op(const TableDev& t, uint8_t op, typename
         }
         if constexpr (!L::IS_MAP) rfound = true;
     } else if (op == OP_REMOVE) {
-        if (found && gl == 0)
-            __hip_atomic_fetch_and(valid_word<L>(t, s), ~(1u << slot), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+        if (found && gl == 0) {
+            if constexpr (COH)
+                st_coh(valid_word<L>(t, s), valid & ~(1u << slot));
+            else
+                __hip_atomic_fetch_and(valid_word<L>(t, s), ~(1u << slot), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+        }
     }
     const bool need_bump = need_alloc;
     // Wave-aggregated bump allocation: one atomic for every group that needs
@@ -329,16 +333,28 @@ __device__ __forceinline__ void apply_op(const TableDev& t, uint8_t op, typename
         }
     }
     if (ins && gl == 0) {
-        // Key and value, then the valid bit, then the link: each step
-        // completes before the next is issued.
         st_coh(key_slot<L>(t, ins_s, ins_slot), key);
         if constexpr (L::IS_MAP) st_coh(val_slot<L>(t, ins_s, ins_slot), val);
-        wait_vm();
-        __hip_atomic_fetch_or(valid_word<L>(t, ins_s), 1u << ins_slot, __ATOMIC_RELAXED,
-                              __HIP_MEMORY_SCOPE_AGENT);
-        if (fresh) {
+        if constexpr (COH) {
+            // Under the bucket's exclusive lock: every other op on this chain
+            // (gets included) waits for the unlock, before which the caller
+            // completes these stores, and only the lock holder writes the
+            // chain's valid words.  So the new bitmap is a plain store of the
+            // one read in the walk, and the stores need no order among them
+            // (an atomic OR issued right behind the key store to the same
+            // line costs more than it saves).
+            st_coh(valid_word<L>(t, ins_s), (fresh ? 0u : valid) | (1u << ins_slot));
+            if (fresh) st_coh(next_word<L>(t, s), (uint32_t)ins_s);
+        } else {
+            // Key and value, then the valid bit, then the link: each step
+            // completes before the next is issued.
             wait_vm();
-            st_coh(next_word<L>(t, s), (uint32_t)ins_s);
+            __hip_atomic_fetch_or(valid_word<L>(t, ins_s), 1u << ins_slot, __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT);
+            if (fresh) {
+                wait_vm();
+                st_coh(next_word<L>(t, s), (uint32_t)ins_s);
+            }
         }
     }
 }
