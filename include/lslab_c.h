@@ -249,8 +249,11 @@ int lslab_dump(lslab_table* t, void* keys_out, void* vals_out, uint64_t cap, uin
  * LSLAB_HASH_IDENTITY (bucket = the key's first 8 bytes) or LSLAB_HASH_MIX.
  * Inputs and outputs are DEVICE arrays: keys n * key_bytes, values n *
  * value_bytes, out_found n bytes, out values n * value_bytes (either output
- * may be NULL).  get reads without locks; put / remove / mixed lock their
- * buckets.  A set reports insert as true, like lslab_set_insert.
+ * may be NULL).  get reads without locks; put / remove lock their buckets;
+ * mixed runs its gets without locks against the table as it was before the
+ * call, then its mutations under the locks (per key the serial order [gets]
+ * [mutations], as the two-phase engine of the narrow tables).  A set reports
+ * insert as true, like lslab_set_insert.
  * stats: slab_bytes counts the tag line plus the 30 slots' arena bytes. */
 typedef struct lslab_wide lslab_wide;
 int lslab_wide_create(lslab_wide** out, int key_bytes, int value_bytes, unsigned n_log2,

@@ -18,9 +18,11 @@
 // words l, l+8, l+16, l+24 of the key and of the value.  A probe loads the
 // slab line (four words per lane), ORs together the slots whose tag matches,
 // and confirms each candidate by comparing the key arena entry four words per
-// lane (coalesced reads, with the value read beside it).  Mutations and gets of mixed calls
-// hold the bucket's reader/writer lock (the reference protocol); read-only
-// calls take no locks.
+// lane (coalesced reads, with the value read beside it).  Mutations hold the
+// bucket's reader/writer lock exclusively (the reference protocol); gets take
+// no locks: get calls, and the first pass of a mixed call, which runs the
+// call's gets against the table as it was before the call; its second pass
+// then runs the mutations.
 //
 // Hash.  h = mix64(XOR over key words l of mix64((l << 32 | word_l) ^ SEED));
 // tag = low 32 bits of h.  Bucket = the key's first 8 bytes (identity, as the
@@ -236,19 +238,26 @@ __device__ __forceinline__ void wide_apply(const WideDev& t, uint8_t op, uint64_
     }
 }
 
-// One 8-lane group per op (lane l holds words l, l+8, l+16, l+24 of the key,
-// the value and a probed slab line).  LOCKED: every op holds its bucket's lock (shared
-// for gets); otherwise (read-only calls) no locks.
+// Which ops of the stream a launch runs: all, only the gets, or only the
+// mutations (the two passes of a mixed call).
+enum : int { SEL_ALL = 0, SEL_GETS = 1, SEL_MUTS = 2 };
+
+// Op i of the stream on one 8-lane group (lane l holds words l, l+8, l+16,
+// l+24 of the key, the value and a probed slab line); i >= n or an op that
+// sel leaves out makes the group idle.  LOCKED: every op holds its bucket's
+// lock (shared for gets); otherwise (calls or passes without mutations) no
+// locks.  Every group of the wave calls it together (the lock loop is
+// wave-uniform).
 template <bool LOCKED>
-__global__ __launch_bounds__(256) void k_wide(WideDev t, WideOps o, uint64_t n) {
+__device__ __forceinline__ void wide_op(const WideDev& t, const WideOps& o, uint64_t i,
+                                        uint64_t n, int sel) {
     const int gl = wlane();
-    const uint64_t i = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WG;
-    const bool act = i < n;
     uint8_t op = OP_GET;
-    if (act) {
+    if (i < n) {
         op = o.ops ? o.ops[i] : o.uop;
         if (op > OP_REMOVE) op = OP_GET;
     }
+    const bool act = i < n && (sel == SEL_ALL || (sel == SEL_GETS) == (op == OP_GET));
     uint32_t kw_[WPL] = {0u, 0u, 0u, 0u}, vw_[WPL] = {0u, 0u, 0u, 0u};
     if (act) ld_words<false>(o.keys + i * (uint64_t)t.kw, t.kw, kw_);
     if (act && op == OP_PUT && o.vals) ld_words<false>(o.vals + i * (uint64_t)t.vw, t.vw, vw_);
@@ -322,6 +331,30 @@ __global__ __launch_bounds__(256) void k_wide(WideDev t, WideOps o, uint64_t n) 
 #pragma unroll
         for (int k = 0; k < WPL; ++k)
             if (word_of(k) < t.vw) o.out_vals[i * (uint64_t)t.vw + word_of(k)] = old[k];
+    }
+}
+
+// One op per group.
+template <bool LOCKED>
+__global__ __launch_bounds__(256) void k_wide(WideDev t, WideOps o, uint64_t n, int sel) {
+    wide_op<LOCKED>(t, o, (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WG, n, sel);
+}
+
+// The mutation pass of a mixed call: each wave reads the opcodes of 64
+// consecutive ops, and its 8 groups take the mutations among them in rounds
+// of 8, so a stream of mostly gets does not start one idle group per get.
+__global__ __launch_bounds__(256) void k_wide_muts(WideDev t, WideOps o, uint64_t n) {
+    const uint64_t base = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / 64u * 64u;
+    const int lane = (int)__lane_id();
+    uint8_t op = OP_GET;
+    if (base + lane < n) op = o.ops[base + lane];
+    uint64_t mask = __ballot(op == OP_PUT || op == OP_REMOVE);  // wave-uniform
+    const int grp = lane / WG;
+    while (mask) {
+        uint64_t mine = mask;
+        for (int k = 0; k < grp && mine; ++k) mine &= mine - 1u;
+        wide_op<true>(t, o, mine ? base + (uint64_t)__builtin_ctzll(mine) : n, n, SEL_MUTS);
+        for (int k = 0; k < 64 / WG && mask; ++k) mask &= mask - 1u;
     }
 }
 
@@ -405,16 +438,29 @@ bool capturing(hipStream_t s) {
 // Key and value records are read and written as 32-bit words.
 bool aligned4(const void* p) { return ((uintptr_t)p & 3u) == 0; }
 
+// locked: the call has mutations.  A mixed call (per-op opcodes) runs in two
+// passes: its gets lock-free against the table as it was before the call,
+// then its mutations under the bucket locks.  Per key the results are those
+// of the serial order [gets][mutations in lock order], which a batch allows;
+// a batch without same-key conflicts gets exactly the sequential replay's.
 int run(lslab_wide* w, const WideOps& o, uint64_t n, bool locked, hipStream_t s) {
     if (n == 0) return LSLAB_OK;
     if (n > (1ull << 31) || !aligned4(o.keys) || !aligned4(o.vals) || !aligned4(o.out_vals))
         return LSLAB_ERR_INVALID;
     if (!capturing(s)) WCHK(hipStreamWaitEvent(s, w->order_ev, 0));
     const uint64_t blocks = (n * WG + 255) / 256;
-    if (locked)
-        hipLaunchKernelGGL(k_wide<true>, dim3((unsigned)blocks), dim3(256), 0, s, w->dev, o, n);
-    else
-        hipLaunchKernelGGL(k_wide<false>, dim3((unsigned)blocks), dim3(256), 0, s, w->dev, o, n);
+    if (locked && o.ops) {
+        hipLaunchKernelGGL(k_wide<false>, dim3((unsigned)blocks), dim3(256), 0, s, w->dev, o, n,
+                           (int)SEL_GETS);
+        hipLaunchKernelGGL(k_wide_muts, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, w->dev,
+                           o, n);
+    } else if (locked) {
+        hipLaunchKernelGGL(k_wide<true>, dim3((unsigned)blocks), dim3(256), 0, s, w->dev, o, n,
+                           (int)SEL_ALL);
+    } else {
+        hipLaunchKernelGGL(k_wide<false>, dim3((unsigned)blocks), dim3(256), 0, s, w->dev, o, n,
+                           (int)SEL_ALL);
+    }
     WCHK(hipGetLastError());
     if (!capturing(s)) WCHK(hipEventRecord(w->order_ev, s));
     return LSLAB_OK;
