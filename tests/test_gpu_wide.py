@@ -201,3 +201,45 @@ def test_wide_rejects_misaligned_records(lslab, oracle_mod):
     f, _ = w.get(raw[:160].view(10, 16))
     assert int(f.sum()) == 0
     assert w.status() == 0
+
+
+@pytest.mark.parametrize("vb", [8, 0])
+def test_wide_mixed_gets_see_the_table_before_the_call(lslab, oracle_mod, vb):
+    """A mixed call runs its gets against the table as it was before the call
+    and its mutations after them: per key the serial order [gets][mutations],
+    wherever the get sits in the batch."""
+    say("wide_mixed_gets_first", vb)
+    w = lslab.Wide(24, vb, 4, 100)
+    keys = [struct.pack("<QQQ", i, i * 7, i * 13) for i in range(1, 41)]
+    v0 = [struct.pack("<Q", 1000 + i) for i in range(40)] if vb else None
+    w.put(dev_rows(keys[:20], 24), dev_rows(v0[:20], 8) if vb else None)  # keys 0..19 present
+    # per key: a get before and after a put (keys 0..9 present, 20..29 absent)
+    # and a get before and after a remove (keys 10..19 present)
+    ops, ks, vs = [], [], []
+    for j in list(range(10)) + list(range(20, 30)):
+        ops += [OP_GET, OP_PUT, OP_GET]
+        ks += [keys[j]] * 3
+        vs += [struct.pack("<Q", 5000 + j)] * 3
+    for j in range(10, 20):
+        ops += [OP_GET, OP_REMOVE, OP_GET]
+        ks += [keys[j]] * 3
+        vs += [bytes(8)] * 3
+    f, out = w.mixed(torch.tensor(ops, dtype=torch.uint8).cuda(), dev_rows(ks, 24),
+                     dev_rows(vs, 8) if vb else None)
+    f = f.cpu().numpy()
+    outs = host_rows(out) if vb else None
+    for t in range(30):
+        j = (list(range(10)) + list(range(20, 30)) + list(range(10, 20)))[t]
+        present = j < 20
+        for pos in (0, 2):  # both gets of the key see the state before the call
+            assert f[3 * t + pos] == (1 if present else 0), (j, pos)
+            if vb:
+                assert outs[3 * t + pos] == (v0[j] if present else bytes(8)), (j, pos)
+    assert w.status() == 0
+    f, out = w.get(dev_rows(keys[:30], 24))
+    f = f.cpu().numpy()
+    assert list(f) == [1] * 10 + [0] * 10 + [1] * 10  # puts landed, removes too
+    if vb:
+        got = host_rows(out)
+        assert got[:10] == [struct.pack("<Q", 5000 + j) for j in range(10)]
+        assert got[20:30] == [struct.pack("<Q", 5000 + j) for j in range(20, 30)]
