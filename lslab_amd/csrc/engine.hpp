@@ -439,6 +439,25 @@ __device__ __forceinline__ uint32_t gt_find_or_insert(const TableDev& t, uint64_
     return 0xffffffffu;  // unreachable while a sub-batch holds <= capacity/2 ops
 }
 
+// gt_find_or_insert with one compare-and-swap per probe (an empty slot to the
+// key), which claims the slot or returns what it holds -- the key itself or
+// another key -- in one round trip.  Used with wavefront combining (+2% on
+// the headline); block combining keeps the load-first probe, which keeps
+// read-modify-writes off the slot of a key that many chains exchange on.
+__device__ __forceinline__ uint32_t gt_claim_cas(const TableDev& t, uint64_t key) {
+    if (key == GT_EMPTY) return t.gt_mask + 1u;  // dedicated entry for the all-ones key
+    uint32_t h = (uint32_t)mix64(key) & t.gt_mask;
+    for (uint32_t probe = 0; probe <= t.gt_mask; ++probe) {
+        uint64_t e = GT_EMPTY;
+        if (__hip_atomic_compare_exchange_strong(&t.gt_key[h], &e, key, __ATOMIC_RELAXED,
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
+            e == key)
+            return h;
+        h = (h + 1u) & t.gt_mask;
+    }
+    return 0xffffffffu;  // unreachable while a sub-batch holds <= capacity/2 ops
+}
+
 // Block-level combining table for one phase-R round.  Two tables alternate
 // by round parity: the entries a round uses are cleared in that round, and a
 // table is reused two rounds (two block barriers) later.
@@ -491,9 +510,10 @@ __device__ __forceinline__ uint32_t comb_slot(CombTab<BT>& c, uint64_t key) {
 // The one global step of a chain: find or insert the key's combining-table
 // entry and swap the chain's last op (index + 1) into its head.  pred gets
 // the previous head: the last op of the chain exchanged before, or 0.
+template <bool CAS_FIRST = false>
 __device__ __forceinline__ uint32_t gt_exchange(const TableDev& t, uint64_t key, uint32_t v,
                                                 uint32_t& pred) {
-    const uint32_t g = gt_find_or_insert(t, key);
+    const uint32_t g = CAS_FIRST ? gt_claim_cas(t, key) : gt_find_or_insert(t, key);
     if (g != 0xffffffffu) {
         pred = __hip_atomic_exchange(&t.gt_head[g], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else {
@@ -691,7 +711,7 @@ __device__ __forceinline__ void phase_r_op(const TableDev& t, const OpsDev& o, c
             }
         }
     } else if (rep) {
-        g = gt_exchange(t, kk, li + 1u, fpred);
+        g = gt_exchange<true>(t, kk, li + 1u, fpred);
     }
     fpred = __shfl(fpred, my_last, 64);
     if constexpr (BC) fdefer = __shfl((int)fdefer, my_last, 64) != 0;
