@@ -129,9 +129,9 @@ __device__ __forceinline__ void wide_find(const WideDev& t, uint64_t b, uint32_t
     const uint64_t nslabs = t.nbuckets + t.pool;
     for (uint64_t steps = 0;; ++steps) {
         uint32_t w[WPL];
-        ld_words<COH>(t.slabs + s * 32, 32, w);  // lanes 6 and 7 hold next (word 30), valid (31)
-        const uint32_t next = __shfl(w[3], WNEXT - WG * 3, WG);
-        const uint32_t valid = __shfl(w[3], WVALID - WG * 3, WG);
+        ld_words<COH>(t.slabs + s * 32, 32, w);  // word x sits in w[x / WG] of lane x % WG
+        const uint32_t next = __shfl(w[WNEXT / WG], WNEXT % WG, WG);
+        const uint32_t valid = __shfl(w[WVALID / WG], WVALID % WG, WG);
         uint32_t mine = 0;
 #pragma unroll
         for (int k = 0; k < WPL; ++k) {
