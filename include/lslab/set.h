@@ -5,7 +5,11 @@
  * Mirrors the reference's set (include/lslab/set.h:21-123 of the reference):
  * contains / insert / remove batches over device pointers, bool results.
  * insert reports true for present and newly inserted keys alike.  The
- * thrust::device_vector overloads become a (pointer, size) pair.
+ * reference's thrust::device_vector overloads (set.h:98-123 of the reference)
+ * are kept without a Thrust dependency: contains / insert / remove also take
+ * any pair of device vectors whose data() is a device-pointer wrapper with
+ * get(), thrust::device_vector<K> and thrust::device_vector<bool> among them.
+ * Host containers, whose data() is a raw pointer, do not match.
  */
 #pragma once
 
@@ -79,6 +83,27 @@ public:
               "set::remove");
     }
 
+    // Device-vector forms: keys.size() ops, results into output (which must
+    // hold at least as many).
+    template <int block_size = 256, typename KV, typename BV>
+    auto contains(const KV& keys, BV& output, hipStream_t stream = 0)
+        -> decltype((void)keys.data().get(), (void)output.data().get()) {
+        check_room(keys.size(), output.size());
+        contains<block_size>(keys.data().get(), output.data().get(), keys.size(), stream);
+    }
+    template <int block_size = 256, typename KV, typename BV>
+    auto insert(const KV& keys, BV& output, hipStream_t stream = 0)
+        -> decltype((void)keys.data().get(), (void)output.data().get()) {
+        check_room(keys.size(), output.size());
+        insert<block_size>(keys.data().get(), output.data().get(), keys.size(), stream);
+    }
+    template <int block_size = 256, typename KV, typename BV>
+    auto remove(const KV& keys, BV& output, hipStream_t stream = 0)
+        -> decltype((void)keys.data().get(), (void)output.data().get()) {
+        check_room(keys.size(), output.size());
+        remove<block_size>(keys.data().get(), output.data().get(), keys.size(), stream);
+    }
+
     void mixed(const uint8_t* ops, const K* keys, bool* output, size_t size,
                hipStream_t stream = 0) {
         check(lslab_set_mixed(t_, ops, keys, reinterpret_cast<uint8_t*>(output), size, stream),
@@ -105,6 +130,10 @@ public:
     lslab_table* handle() { return t_; }
 
 private:
+    static void check_room(size_t n, size_t room) {
+        if (room < n) check(LSLAB_ERR_INVALID, "set: output shorter than keys");
+    }
+
     lslab_table* t_ = nullptr;
     unsigned n_log_2_;
 };

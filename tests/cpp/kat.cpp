@@ -3,7 +3,8 @@
 //   GetPutTestGroupAllocator   test/slabunified_test.cu:156-205 (the foreign
 //                              allocator is a caller-owned arena: two maps'
 //                              slab arrays carved from one hipMalloc)
-//   ContainsInsertRemoveTest   test/set_test.cu:65-106
+//   ContainsInsertRemoveTest   test/set_test.cu:65-106, over device pointers and
+//                              over thrust::device_vector (set.h:98-123)
 //   PutRemoveTest_128B         test/slabunified_test.cu.old:386-474 (128-B keys
 //   PutRemoveTest_128Bto128B   test/slabunified_test.cu.old:476-541  on lslab::wide_map)
 // GetPutTest also runs on a table created with lslab::mix_hash (the
@@ -15,6 +16,9 @@
 #include <stdexcept>
 #include <string>
 #include <vector>
+
+#include <thrust/device_vector.h>
+#include <thrust/host_vector.h>
 
 #include <lslab/map.h>
 #include <lslab/set.h>
@@ -141,6 +145,40 @@ static void contains_insert_remove(int engine) {
     ASSERT(s.status() == LSLAB_OK, "device status");
 }
 
+// The same test through the set's thrust::device_vector overloads.
+static void contains_insert_remove_device_vector(int engine) {
+    constexpr int size = 1024;
+    lslab_tuning tu{};
+    tu.engine = engine;
+    lslab::set<int> s(10, 10000, &tu);
+    std::vector<int> hk(size);
+    for (int i = 0; i < size; ++i) hk[i] = i + 1;
+    thrust::device_vector<int> keys(hk.begin(), hk.end());
+    thrust::device_vector<bool> res1(size), res2(size);
+    for (int rep = 0; rep < 100; rep++) {
+        s.insert(keys, res1);
+        s.contains(keys, res2);
+        HIPOK(hipDeviceSynchronize());
+        thrust::host_vector<bool> h1 = res1, h2 = res2;
+        for (int i = 0; i < size; ++i) ASSERT(h1[i] && h2[i], "insert / contains");
+        s.remove(keys, res1);
+        s.contains(keys, res2);
+        HIPOK(hipDeviceSynchronize());
+        h1 = res1;
+        h2 = res2;
+        for (int i = 0; i < size; ++i) ASSERT(h1[i] && !h2[i], "remove / contains after remove");
+    }
+    bool threw = false;
+    try {
+        thrust::device_vector<bool> small(size - 1);
+        s.contains(keys, small);
+    } catch (const lslab::error&) {
+        threw = true;
+    }
+    ASSERT(threw, "output shorter than keys is refused");
+    ASSERT(s.status() == LSLAB_OK, "device status");
+}
+
 // The legacy test's 128-byte Key: i in the first 8 bytes, zeros after
 // (test/slabunified_test.cu.old:344-364).
 struct Key128 {
@@ -190,6 +228,8 @@ int main() {
             std::printf("GetPutTestGroupAllocator engine %d ok\n", engine);
             contains_insert_remove(engine);
             std::printf("ContainsInsertRemoveTest engine %d ok\n", engine);
+            contains_insert_remove_device_vector(engine);
+            std::printf("ContainsInsertRemoveTest device_vector engine %d ok\n", engine);
             get_put<lslab::mix_hash<int>>(10000, engine);
             std::printf("GetPutTest mix_hash engine %d ok\n", engine);
         }

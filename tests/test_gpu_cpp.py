@@ -18,4 +18,4 @@ def test_cpp_kats(lslab):
     r = subprocess.run([exe], capture_output=True, text=True, timeout=110)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
-    assert r.stdout.count(" ok") == 10
+    assert r.stdout.count(" ok") == 12
