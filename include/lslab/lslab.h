@@ -99,16 +99,23 @@ constexpr bool slab_word() {
 }
 
 /// The reference chooses the CTA size per call as a template argument
-/// (map.h:189-220).  Wave64 kernels here take 64..256 threads per block.
+/// (map.h:189-220, default 256).  Here the default, 0, keeps the table's own
+/// block size (set at construction or by set_tuning); an explicit size (wave64
+/// kernels take 64..256 threads per block, so it is clamped to that) becomes
+/// the table's block size, as set_tuning would make it -- a host-side change
+/// of shared table state, so callers that share a table across host threads
+/// pass sizes through set_tuning instead.
 template <int block_size>
 inline void apply_block_size(lslab_table* t) {
-    static_assert(block_size >= 32 && block_size <= 1024, "block_size");
-    constexpr int bt = block_size < 64 ? 64 : (block_size > 256 ? 256 : block_size);
-    lslab_tuning tu;
-    check(lslab_get_tuning(t, &tu), "get_tuning");
-    if (tu.block_threads != bt) {
-        tu.block_threads = bt;
-        check(lslab_set_tuning(t, &tu), "set_tuning");
+    static_assert(block_size == 0 || (block_size >= 32 && block_size <= 1024), "block_size");
+    if constexpr (block_size != 0) {
+        constexpr int bt = block_size < 64 ? 64 : (block_size > 256 ? 256 : block_size);
+        lslab_tuning tu;
+        check(lslab_get_tuning(t, &tu), "get_tuning");
+        if (tu.block_threads != bt) {
+            tu.block_threads = bt;
+            check(lslab_set_tuning(t, &tu), "set_tuning");
+        }
     }
 }
 }  // namespace detail

@@ -67,14 +67,14 @@ public:
 
     /// Batch of puts from pair<K,V> (reference map.h:189-192); output[i] is
     /// the previous value, 0 if the key was absent.
-    template <int block_size = 256>
+    template <int block_size = 0>
     void put(key_value<K, V>* operations, V* output, size_t size, hipStream_t stream = 0) {
         detail::apply_block_size<block_size>(t_);
         check(lslab_map_put_pairs(t_, operations, output, size, stream), "map::put");
     }
 
     /// Batch of puts from separate arrays (reference map.h:198-201).
-    template <int block_size = 256>
+    template <int block_size = 0>
     void put(K* operations_keys, V* operations_values, V* output, size_t size,
              hipStream_t stream = 0) {
         detail::apply_block_size<block_size>(t_);
@@ -83,7 +83,7 @@ public:
     }
 
     /// Batch of gets (reference map.h:208-211): output[i] = {found, value}.
-    template <int block_size = 256>
+    template <int block_size = 0>
     void get(K* operations, found_value<V>* output, size_t size, hipStream_t stream = 0) {
         detail::apply_block_size<block_size>(t_);
         check(lslab_map_get(t_, operations, output, size, stream), "map::get");
@@ -92,7 +92,7 @@ public:
     /// Batch of removes: output[i] = {was present, removed value}.  The
     /// reference traversal supports REMOVE (detail/traverse.h:88-91) but its
     /// map does not expose it.
-    template <int block_size = 256>
+    template <int block_size = 0>
     void remove(K* operations, found_value<V>* output, size_t size, hipStream_t stream = 0) {
         detail::apply_block_size<block_size>(t_);
         check(lslab_map_remove(t_, operations, output, size, stream), "map::remove");

@@ -62,21 +62,21 @@ public:
         if (t_) lslab_destroy(t_);
     }
 
-    template <int block_size = 256>
+    template <int block_size = 0>
     void contains(const K* keys, bool* output, size_t size, hipStream_t stream = 0) {
         detail::apply_block_size<block_size>(t_);
         check(lslab_set_contains(t_, keys, reinterpret_cast<uint8_t*>(output), size, stream),
               "set::contains");
     }
 
-    template <int block_size = 256>
+    template <int block_size = 0>
     void insert(const K* keys, bool* output, size_t size, hipStream_t stream = 0) {
         detail::apply_block_size<block_size>(t_);
         check(lslab_set_insert(t_, keys, reinterpret_cast<uint8_t*>(output), size, stream),
               "set::insert");
     }
 
-    template <int block_size = 256>
+    template <int block_size = 0>
     void remove(const K* keys, bool* output, size_t size, hipStream_t stream = 0) {
         detail::apply_block_size<block_size>(t_);
         check(lslab_set_remove(t_, keys, reinterpret_cast<uint8_t*>(output), size, stream),
@@ -85,19 +85,19 @@ public:
 
     // Device-vector forms: keys.size() ops, results into output (which must
     // hold at least as many).
-    template <int block_size = 256, typename KV, typename BV>
+    template <int block_size = 0, typename KV, typename BV>
     auto contains(const KV& keys, BV& output, hipStream_t stream = 0)
         -> decltype((void)keys.data().get(), (void)output.data().get()) {
         check_room(keys.size(), output.size());
         contains<block_size>(keys.data().get(), output.data().get(), keys.size(), stream);
     }
-    template <int block_size = 256, typename KV, typename BV>
+    template <int block_size = 0, typename KV, typename BV>
     auto insert(const KV& keys, BV& output, hipStream_t stream = 0)
         -> decltype((void)keys.data().get(), (void)output.data().get()) {
         check_room(keys.size(), output.size());
         insert<block_size>(keys.data().get(), output.data().get(), keys.size(), stream);
     }
-    template <int block_size = 256, typename KV, typename BV>
+    template <int block_size = 0, typename KV, typename BV>
     auto remove(const KV& keys, BV& output, hipStream_t stream = 0)
         -> decltype((void)keys.data().get(), (void)output.data().get()) {
         check_room(keys.size(), output.size());

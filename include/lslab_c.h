@@ -103,7 +103,7 @@ typedef struct {
     uint64_t slabs_used;   /* pool slabs handed out (min(bump pointer, pool)) */
     uint64_t present;      /* keys whose valid bit is set */
     uint64_t max_chain;    /* longest bucket chain, in slabs */
-    uint64_t slab_bytes;   /* bytes per slab */
+    uint64_t slab_bytes;   /* device bytes per slab (a u32 set: 128-B key line + 4-B link) */
     uint64_t device_bytes; /* bytes of device memory the table holds */
 } lslab_stats;
 

@@ -136,14 +136,15 @@ static int alloc_scratch(lslab_table* t) {
     return LSLAB_OK;
 }
 
+// Device bytes per slab (a u32 set's next link included).
 static uint64_t layout_slab_bytes(TableType type) {
     switch (type) {
-        case T_MAP_32_32: return L_M3232::SLAB_BYTES;
-        case T_MAP_32_64: return L_M3264::SLAB_BYTES;
-        case T_MAP_64_32: return L_M6432::SLAB_BYTES;
-        case T_MAP_64_64: return L_M6464::SLAB_BYTES;
-        case T_SET_32: return L_S32::SLAB_BYTES;
-        case T_SET_64: return L_S64::SLAB_BYTES;
+        case T_MAP_32_32: return L_M3232::FOOTPRINT;
+        case T_MAP_32_64: return L_M3264::FOOTPRINT;
+        case T_MAP_64_32: return L_M6432::FOOTPRINT;
+        case T_MAP_64_64: return L_M6464::FOOTPRINT;
+        case T_SET_32: return L_S32::FOOTPRINT;
+        case T_SET_64: return L_S64::FOOTPRINT;
     }
     return 0;
 }
@@ -195,7 +196,7 @@ static int create_common(lslab_table** out, TableType type, bool is_map, int kb,
     t->pool = pool;
     t->tun = tu;
     dispatch(t, [&](auto l) {
-        t->slab_bytes = decltype(l)::SLAB_BYTES;
+        t->slab_bytes = decltype(l)::FOOTPRINT;
         return 0;
     });
     const uint64_t nslabs = t->nbuckets + pool;

@@ -2,21 +2,25 @@
 // gfx950.
 //
 // Slab layout (one slab = one key block, then one value block for maps):
-//   key block: 32 slots of K.  Slots SLOT0..SLOT0+CAP-1 hold keys.
+//   key block: 32 slots of K.  31 slots hold keys, one holds meta.
 //     K = u64        : slot 0 = valid bitmap (low 32 bits) | next slab (high 32),
 //                      keys in slots 1..31
-//     K = u32, map   : slot 31 = valid bitmap; next slab lives in value slot 31
-//     K = u32, set   : slot 30 = next slab, slot 31 = valid bitmap (CAP = 30)
+//     K = u32        : keys in slots 0..30, slot 31 = valid bitmap; the next
+//                      link is value slot 31 (maps) or, for sets, which have
+//                      no value block, entry s of a u32 link array that
+//                      follows the slab array
 //   value block (maps): 32 slots of V; value slot i belongs to key slot i.
 //   Valid bit i belongs to key slot i.
-// This keeps the reference's 31 keys per slab and its valid-bit semantics
-// (detail/slab_node.h:13-35) with a 32-bit slab index in place of the 8-byte
-// pointer, so a u64 key block is exactly 256 B (two 128-B lines) and a u32 key
-// block one line.  The u64 meta word sits in the first line together with
-// the first 15 keys: keys fill the lowest free slot first, so a probe of a
-// slab holding at most 15 keys, or of a key among its first 15, reads one
-// line (see PW).  Slabs 0..2^n-1 are the bucket heads; pool slabs follow
-// them in the same array, so next == 0 means end of chain.
+// Every layout keeps the reference's 31 keys per slab and its valid-bit
+// semantics (detail/slab_node.h:13-35) with a 32-bit slab index in place of
+// the 8-byte pointer, so a u64 key block is exactly 256 B (two 128-B lines)
+// and a u32 key block one line.  The u64 meta word sits in the first line
+// together with the first 15 keys: keys fill the lowest free slot first, so a
+// probe of a slab holding at most 15 keys, or of a key among its first 15,
+// reads one line (see PW).  A u32 probe reads the key line; its next link is
+// read only when the key is not in the slab.  Slabs 0..2^n-1 are the bucket
+// heads; pool slabs follow them in the same array, so next == 0 means end of
+// chain.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -45,13 +49,16 @@ struct Layout {
     static constexpr uint32_t KB = 32u * sizeof(K);
     static constexpr uint32_t VB = MAP ? 32u * sizeof(V) : 0u;
     static constexpr uint32_t SLAB_BYTES = KB + VB;
-    static constexpr int CAP = (sizeof(K) == 4 && !MAP) ? 30 : 31;
+    static constexpr int CAP = 31;
     static constexpr int SLOT0 = (sizeof(K) == 8) ? 1 : 0;  // first key slot
     static constexpr uint32_t CAPMASK = ((1u << CAP) - 1u) << SLOT0;  // key slots
+    // u32 sets keep their next links in an array after the slab array.
+    static constexpr bool SIDE_NEXT = sizeof(K) == 4 && !MAP;
     // Byte offsets inside a slab.
     static constexpr uint32_t VALID_OFF = (sizeof(K) == 8) ? 0u : 31u * 4u;
-    static constexpr uint32_t NEXT_OFF = (sizeof(K) == 8) ? 4u
-                                         : (MAP ? KB + 31u * sizeof(V) : 30u * 4u);
+    static constexpr uint32_t NEXT_OFF = (sizeof(K) == 8) ? 4u : (MAP ? KB + 31u * sizeof(V) : 0u);
+    // Device bytes per slab, its share of the link array included.
+    static constexpr uint32_t FOOTPRINT = SLAB_BYTES + (SIDE_NEXT ? 4u : 0u);
 };
 
 // Device view of one table (passed by value into kernels).
@@ -65,7 +72,8 @@ struct RepEnt {
 };
 
 struct TableDev {
-    char* slabs;           // (nbuckets + pool) * SLAB_BYTES, 256-B aligned
+    char* slabs;           // (nbuckets + pool) * SLAB_BYTES, 256-B aligned, then
+                           // (u32 sets) one u32 next link per slab
     uint32_t* locks;       // one RW lock word per bucket
     uint64_t* alloc_ctr;   // bump pointer into the pool (slabs handed out)
     uint32_t* status;      // sticky ST_* bits
@@ -226,7 +234,10 @@ __device__ __forceinline__ uint32_t* valid_word(const TableDev& t, uint64_t s) {
 }
 template <class L>
 __device__ __forceinline__ uint32_t* next_word(const TableDev& t, uint64_t s) {
-    return reinterpret_cast<uint32_t*>(slab_ptr<L>(t, s) + L::NEXT_OFF);
+    if constexpr (L::SIDE_NEXT)
+        return reinterpret_cast<uint32_t*>(t.slabs + (t.nbuckets + t.pool) * L::SLAB_BYTES) + s;
+    else
+        return reinterpret_cast<uint32_t*>(slab_ptr<L>(t, s) + L::NEXT_OFF);
 }
 template <class L>
 __device__ __forceinline__ typename L::K* key_slot(const TableDev& t, uint64_t s, int i) {
@@ -288,10 +299,7 @@ __device__ __forceinline__ Probe probe_compute(const TableDev& t, uint64_t s, ty
         const uint64_t meta = grp_bcast((uint64_t)mine[0], 0, G);  // slot 0, lane 0
         p.valid = (uint32_t)meta;
         p.next = (uint32_t)(meta >> 32);
-    } else if constexpr (!L::IS_MAP) {
-        p.valid = grp_bcast((uint32_t)mine[31 % N], 31 / N, G);
-        p.next = grp_bcast((uint32_t)mine[30 % N], 30 / N, G);
-    } else {
+    } else {  // u32 keys: the valid word is slot 31, the last lane's last slot
         p.valid = grp_bcast((uint32_t)mine[N - 1], G - 1, G);
         p.next = 0;
     }
@@ -306,7 +314,7 @@ __device__ __forceinline__ Probe probe_compute(const TableDev& t, uint64_t s, ty
         }
     }
     if (p.slot < 0) {
-        if constexpr (sizeof(K) == 4 && L::IS_MAP)
+        if constexpr (sizeof(K) == 4)
             p.next = COH ? ld_coh(next_word<L>(t, s)) : *next_word<L>(t, s);
     }
     return p;

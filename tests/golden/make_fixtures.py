@@ -37,7 +37,7 @@ def make_case(is_map, kb, vb, n_log2, pool, seed):
     universe[:3] = [0, kmax - 1, kmax >> 1]  # width edges, incl. the all-ones key
     universe = np.unique(universe)
     rng.shuffle(universe)
-    cap = 30 if (kb == 4 and not is_map) else 31
+    cap = 31  # keys per slab, every layout (detail/slab_node.h:13-35)
     r = Replay(is_map, kb, vb, n_log2, pool_slabs=pool, slab_keys=cap)
     batches = []
     for b in range(8):

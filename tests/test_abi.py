@@ -74,15 +74,16 @@ def test_cpp_headers_compile():
 
 
 def test_slab_bytes_closed_form(lib):
-    """lslab_slab_bytes: (2^n + pool) slabs of the layout's size; 0 for an
-    invalid shape.  Pure arithmetic, no device call."""
+    """lslab_slab_bytes: (2^n + pool) slabs of the layout's size (a u32 set's
+    128-B key line plus its 4-B next link); 0 for an invalid shape.  Pure
+    arithmetic, no device call."""
     f = lib.lslab_slab_bytes
     assert f(1, 8, 8, 10, 100) == (1024 + 100) * 512
     assert f(1, 4, 4, 3, 0) == 8 * 256
     assert f(1, 4, 8, 3, 5) == 13 * 384
     assert f(1, 8, 4, 3, 5) == 13 * 384
     assert f(0, 8, 0, 4, 1) == 17 * 256
-    assert f(0, 4, 0, 4, 1) == 17 * 128
+    assert f(0, 4, 0, 4, 1) == 17 * 132
     assert f(1, 2, 8, 4, 1) == 0
     assert f(0, 16, 0, 4, 1) == 0
     assert f(1, 8, 8, 31, 1) == 0

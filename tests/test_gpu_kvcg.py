@@ -27,9 +27,8 @@ def test_kvcg_distinct_matches_replay(lslab, oracle_mod, is_map, kb, vb, engine)
     say("kvcg_distinct", is_map, kb, vb, engine)
     rng = np.random.default_rng(21 + engine)
     n_log2, pool = 8, 1 << 13
-    cap = 30 if (not is_map and kb == 4) else 31
     tbl = make(lslab, is_map, kb, vb, n_log2, pool, engine, lanes=8)
-    ref = Replay(is_map, kb, vb, n_log2, pool_slabs=pool, slab_keys=cap)
+    ref = Replay(is_map, kb, vb, n_log2, pool_slabs=pool)
     kv = lslab.Kvcg(tbl, hot_capacity=512, max_batch=4096, threads=4)
     space = _mask(rng.permutation(40000).astype(np.uint64) * np.uint64(0x9E3779B97F4A7C15)
                   + np.uint64(7), kb)

@@ -135,9 +135,8 @@ def test_phase_separated_bit_exact(lslab, oracle_mod, is_map, kb, vb, engine, la
     say("phase_separated", is_map, kb, vb, engine, lanes)
     rng = np.random.default_rng(100 + kb + vb + lanes + 7 * engine)
     n_log2 = 4
-    cap = 30 if (kb == 4 and not is_map) else 31
     tbl = make(lslab, is_map, kb, vb, n_log2, 4096, engine, lanes)
-    ref = Replay(is_map, kb, vb, n_log2, pool_slabs=4096, slab_keys=cap)
+    ref = Replay(is_map, kb, vb, n_log2, pool_slabs=4096)
     universe = rng.permutation(np.arange(1, 6001, dtype=np.uint64) * np.uint64(2654435761))
     if kb == 4:
         universe = universe & np.uint64(0xFFFFFFFF)
@@ -246,9 +245,9 @@ def test_two_phase_inserts_one_bucket_pair(lslab, oracle_mod, is_map, kb, vb, la
     keys = np.unique(rng.integers(1, 2**31, size=n + 100, dtype=np.uint64))[:n]
     rng.shuffle(keys)
     vals = rng.integers(1, 2**31, size=n, dtype=np.uint64)
-    cap = 31 if (kb == 8 or is_map) else 30  # keys per slab
+    cap = 31  # keys per slab, every layout
     tbl = make(lslab, is_map, kb, vb, 1, 400, 1, lanes)
-    ref = Replay(is_map, kb, vb, 1, pool_slabs=400, slab_keys=cap)
+    ref = Replay(is_map, kb, vb, 1, pool_slabs=400)
     ops = np.full(n, OP_PUT, np.uint8)
     run_batch(tbl, is_map, kb, vb, ops, keys, vals)
     ref.apply(ops, keys, vals if is_map else None)
@@ -398,8 +397,7 @@ def test_chain_histogram_matches_oracle(lslab, oracle_mod, is_map, kb, vb):
     rng = np.random.default_rng(17)
     n_log2 = 8
     tbl = make(lslab, is_map, kb, vb, n_log2, 1 << 12)
-    cap = 30 if (not is_map and kb == 4) else 31  # keys per slab of the layout
-    ref = Replay(is_map, kb, vb, n_log2, pool_slabs=1 << 12, slab_keys=cap)
+    ref = Replay(is_map, kb, vb, n_log2, pool_slabs=1 << 12)
     # skewed bucket loads: bucket b gets about b/4 keys, some chains span many slabs
     keys = np.concatenate([np.arange(rng.integers(0, 300), dtype=np.uint64) * np.uint64(256)
                            + np.uint64(b) for b in range(256)])
@@ -421,11 +419,10 @@ def test_mix_hash_table(lslab, oracle_mod, is_map, kb, vb, engine):
     the oracle's fmix64 places them, and every result is the replay's."""
     say("mix_hash", is_map, kb, vb, engine)
     n_log2 = 7
-    cap = 30 if (not is_map and kb == 4) else 31
     tun = lslab.tuning(lanes_per_op=4, engine=engine, max_subbatch=1 << 14, hash=lslab.HASH_MIX)
     tbl = (lslab.Map(kb, vb, n_log2, 1 << 12, tun) if is_map
            else lslab.Set(kb, n_log2, 1 << 12, tun))
-    ref = Replay(is_map, kb, vb, n_log2, pool_slabs=1 << 12, slab_keys=cap, mix_hash=True)
+    ref = Replay(is_map, kb, vb, n_log2, pool_slabs=1 << 12, mix_hash=True)
     keys = np.arange(1, 6001, dtype=np.uint64) << np.uint64(n_log2)
     rng = np.random.default_rng(3)
     rng.shuffle(keys)
@@ -517,9 +514,8 @@ def test_reclaim_random_stream(lslab, oracle_mod, is_map, kb, vb, engine, lanes)
     say("reclaim_random", is_map, kb, vb, engine, lanes)
     rng = np.random.default_rng(31 + kb + lanes + engine)
     n_log2, pool = 3, 2048
-    cap = 30 if (not is_map and kb == 4) else 31
     tbl = make_reclaim(lslab, is_map, kb, vb, n_log2, pool, engine, lanes, True)
-    ref = Replay(is_map, kb, vb, n_log2, pool_slabs=pool, slab_keys=cap, reclaim=True)
+    ref = Replay(is_map, kb, vb, n_log2, pool_slabs=pool, reclaim=True)
     universe = np.arange(1, 20001, dtype=np.uint64) * np.uint64(2654435761)
     if kb == 4:
         universe &= np.uint64(0xFFFFFFFF)
@@ -559,9 +555,9 @@ def test_caller_slab_memory(lslab, oracle_mod, is_map, kb, vb, engine):
     table's, and destroying the table leaves the buffer to its owner."""
     say("caller_slab_memory", is_map, kb, vb, engine)
     n_log2, pool = 6, 1 << 10
-    cap = 30 if (not is_map and kb == 4) else 31
     need = lslab.slab_bytes(is_map, kb, vb, n_log2, pool)
-    slab = {(True, 8, 8): 512, (True, 4, 8): 384, (False, 4, 0): 128}[(is_map, kb, vb)]
+    # a u32 set's slab is its 128-B key line plus a 4-B next link
+    slab = {(True, 8, 8): 512, (True, 4, 8): 384, (False, 4, 0): 132}[(is_map, kb, vb)]
     assert need == ((1 << n_log2) + pool) * slab
     arena = torch.full((need + 512,), 0xAB, dtype=torch.uint8, device="cuda")
     tun = lslab.tuning(lanes_per_op=8, engine=engine, max_subbatch=1 << 14)
@@ -575,7 +571,7 @@ def test_caller_slab_memory(lslab, oracle_mod, is_map, kb, vb, engine):
     with pytest.raises(lslab.LslabError):
         mk(arena[1:need + 1])
     tbl = mk(arena[256:256 + need])
-    ref = Replay(is_map, kb, vb, n_log2, pool_slabs=pool, slab_keys=cap)
+    ref = Replay(is_map, kb, vb, n_log2, pool_slabs=pool)
     rng = np.random.default_rng(11)
     for step in range(6):
         keys = rng.integers(1, 20000, size=5000, dtype=np.uint64)

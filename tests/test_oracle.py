@@ -91,8 +91,7 @@ def test_brute_force_vs_dict(oracle_mod, is_map, kb, vb):
     ops = rng.choice([OP_GET, OP_PUT, OP_REMOVE], size=n, p=[0.4, 0.4, 0.2]).astype(np.uint8)
     keys = keyspace[rng.integers(0, len(keyspace), size=n)]
     vals = rng.integers(0, 2**64, size=n, dtype=np.uint64)
-    cap = 30 if (kb == 4 and not is_map) else 31
-    r = Replay(is_map, kb, vb, 2, pool_slabs=100000, slab_keys=cap)
+    r = Replay(is_map, kb, vb, 2, pool_slabs=100000)
     f, o, fail = r.apply(ops, keys, vals)
     assert fail == 0
     kmask = 2**32 - 1 if kb == 4 else 2**64 - 1
