@@ -186,6 +186,27 @@ int lslab_exec_batches(lslab_table* t, size_t nbatches, const uint64_t* offsets,
                        const uint8_t* ops, const void* keys, const void* values, void* out,
                        hipStream_t stream);
 
+/* Batched form that allocates nothing (new; no counterpart in the reference).
+ * For callers that cannot allocate device memory at the call -- code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
+ * Same arguments, kernels and results as lslab_exec_batches, but the
+ * batched kernel's scratch (grid-barrier state, combining table and
+ * final-candidate lists) lives in one static device array of the library
+ * (about 7 MiB per GPU, held until the process ends) instead of the table's.
+ * Limits: the table's max_subbatch must be at most
+ * LSLAB_NOALLOC_MAX_SUBBATCH ops and its persistent grid (CUs, or max_blocks
+ * when set) at most LSLAB_NOALLOC_MAX_BLOCKS blocks; otherwise the call
+ * returns LSLAB_ERR_INVALID before any device work.  Every call on a GPU
+ * shares that GPU's array, so calls that overlap on one GPU (other streams,
+ * other tables) race: the caller orders them.  The array keeps the last
+ * call's contents; each call empties the combining table and resets the
+ * barrier with kernels of its own before the batched kernel runs. */
+#define LSLAB_NOALLOC_MAX_SUBBATCH 65536
+#define LSLAB_NOALLOC_MAX_BLOCKS 256
+int lslab_exec_batches_noalloc(lslab_table* t, size_t nbatches, const uint64_t* offsets,
+                               const uint8_t* ops, const void* keys, const void* values,
+                               void* out, hipStream_t stream);
+
 /* Served form (new): a device-resident request loop.  Request r owns ops
  * [req_offsets[r], req_offsets[r+1]) of ops/keys/values/out and arrives
  * arrival_ticks[r] ticks of the device's 100 MHz constant clock after the

@@ -46,8 +46,11 @@ EXPORTS = (
     "lslab_kvcg_sample_hot", "lslab_kvcg_mixed", "lslab_kvcg_flush", "lslab_kvcg_stats_get",
     "lslab_wide_create", "lslab_wide_destroy", "lslab_wide_put", "lslab_wide_get",
     "lslab_wide_remove", "lslab_wide_mixed", "lslab_wide_status", "lslab_wide_stats",
-    "lslab_wide_dump", "lslab_wide_slab_count",
+    "lslab_wide_dump", "lslab_wide_slab_count", "lslab_exec_batches_noalloc",
 )
+
+NOALLOC_MAX_SUBBATCH = 65536
+NOALLOC_MAX_BLOCKS = 256
 
 SERVE_REJECTED = 0xFFFFFFFF
 
@@ -144,6 +147,7 @@ def load():
         "lslab_set_remove": [vp, vp, vp, sz, vp],
         "lslab_set_mixed": [vp, vp, vp, vp, sz, vp],
         "lslab_exec_batches": [vp, sz, vp, vp, vp, vp, vp, vp],
+        "lslab_exec_batches_noalloc": [vp, sz, vp, vp, vp, vp, vp, vp],
         "lslab_status": [vp, vp],
         "lslab_stats_get": [vp, ctypes.POINTER(Stats), vp],
         "lslab_dump": [vp, vp, vp, u64, ctypes.POINTER(ctypes.c_uint64), vp],

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 
 #include "kernels.hpp"
@@ -33,6 +34,63 @@ struct lslab_table {
         hipError_t e_ = (x);                        \
         if (e_ != hipSuccess) return LSLAB_ERR_HIP; \
     } while (0)
+
+// ---------------------------------------------------------------------------
+// Static scratch of lslab_exec_batches_noalloc: the persistent batched
+// kernel's grid-barrier state, combining table and final-candidate lists for
+// sub-batches of at most LSLAB_NOALLOC_MAX_SUBBATCH ops, in one file-scope
+// device array.  Every GPU the library runs on holds its own copy until the
+// process ends.  Host code never passes the array's name to a kernel: each
+// device's address comes from hipGetSymbolAddress, looked up once per device
+// at table creation with that device current.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr uint64_t NA_SUB = LSLAB_NOALLOC_MAX_SUBBATCH;
+constexpr uint64_t NA_GT_CAP = 2 * NA_SUB;  // combining-table entries (power of two)
+constexpr uint64_t NA_BLOCKS = LSLAB_NOALLOC_MAX_BLOCKS;
+constexpr uint64_t na_align(uint64_t x) { return (x + 255) & ~255ull; }
+constexpr uint64_t NA_BAR_OFF = 0;
+constexpr uint64_t NA_GTK_OFF = na_align(NA_BAR_OFF + sizeof(BarState));
+constexpr uint64_t NA_GTH_OFF = na_align(NA_GTK_OFF + (NA_GT_CAP + 1) * sizeof(uint64_t));
+constexpr uint64_t NA_REP_OFF = na_align(NA_GTH_OFF + (NA_GT_CAP + 1) * sizeof(uint32_t));
+constexpr uint64_t NA_REP_LEN = NA_SUB + NA_BLOCKS * PERSIST_THREADS;
+constexpr uint64_t NA_BYTES = na_align(NA_REP_OFF + NA_REP_LEN * sizeof(RepEnt));
+static_assert(NA_BYTES <= 16ull << 20, "static scratch stays under 16 MiB");
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+// Device address of `scratch` per device (nullptr until looked up).
+static constexpr int NA_MAX_DEVICES = 64;
+static std::mutex na_mutex;
+static char* na_ptr[NA_MAX_DEVICES];
+
+// Looks up this device's copy of `scratch` (the device must be current).
+static int noalloc_setup(int device) {
+    if (device < 0 || device >= NA_MAX_DEVICES) return LSLAB_ERR_INVALID;
+    std::lock_guard<std::mutex> g(na_mutex);
+    if (na_ptr[device]) return LSLAB_OK;
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return LSLAB_ERR_HIP;
+    na_ptr[device] = static_cast<char*>(p);
+    return LSLAB_OK;
+}
+
+static char* noalloc_ptr(int device) {
+    if (device < 0 || device >= NA_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> g(na_mutex);
+    return na_ptr[device];
+}
+
+// Empties the static combining table (key = all ones, head = 0) before a
+// call: a kernel, not a memset, so the call can be captured into a graph.
+__global__ void k_noalloc_clear(uint64_t* gt_key, uint32_t* gt_head, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        gt_key[i] = GT_EMPTY;
+        gt_head[i] = 0;
+    }
+}
 
 template <typename F>
 static int dispatch(const lslab_table* t, F&& f) {
@@ -209,6 +267,12 @@ static int create_common(lslab_table** out, TableType type, bool is_map, int kb,
             hipSuccess) {
         delete t;
         return LSLAB_ERR_HIP;
+    }
+    // Set-up of lslab_exec_batches_noalloc: this device's static scratch
+    // address, looked up once per device with the device current.
+    if (int rc0 = noalloc_setup(t->device)) {
+        delete t;
+        return rc0;
     }
     int rc = LSLAB_OK;
     const uint64_t slab_total = nslabs * t->slab_bytes;
@@ -478,6 +542,41 @@ extern "C" int lslab_exec_batches(lslab_table* t, size_t nbatches, const uint64_
     BarState* bar = &t->state->bar;
     rc = dispatch(t, [&](auto l) {
         return Launch<decltype(l)>::batches(ctx(t), o, d, offsets, (uint64_t)nbatches, bar, stream);
+    });
+    if (rc) return rc;
+    return end_call(t, stream);
+}
+
+extern "C" int lslab_exec_batches_noalloc(lslab_table* t, size_t nbatches, const uint64_t* offsets,
+                                          const uint8_t* ops, const void* keys,
+                                          const void* values, void* out, hipStream_t stream) {
+    if (!t || (nbatches && (!offsets || !ops || !keys || !out))) return LSLAB_ERR_INVALID;
+    if (t->is_map && nbatches && !values) return LSLAB_ERR_INVALID;
+    // Sizes the static scratch cannot hold are refused before any device work.
+    if (t->submax > NA_SUB) return LSLAB_ERR_INVALID;
+    const uint64_t blocks = t->tun.max_blocks > 0
+                                ? std::min<uint64_t>((uint64_t)t->tun.max_blocks, t->num_cus)
+                                : (uint64_t)t->num_cus;
+    if (blocks > NA_BLOCKS) return LSLAB_ERR_INVALID;
+    if (nbatches == 0) return LSLAB_OK;
+    char* base = noalloc_ptr(t->device);
+    if (!base) return LSLAB_ERR_HIP;  // set-up did not run on this device
+    int rc = begin_call(t, stream);
+    if (rc) return rc;
+    LaunchCtx c = ctx(t);
+    c.dev.gt_key = reinterpret_cast<uint64_t*>(base + NA_GTK_OFF);
+    c.dev.gt_head = reinterpret_cast<uint32_t*>(base + NA_GTH_OFF);
+    c.dev.gt_mask = (uint32_t)(NA_GT_CAP - 1);
+    c.dev.rep = reinterpret_cast<RepEnt*>(base + NA_REP_OFF);
+    c.dev.rep_cnt = nullptr;  // only the per-call kernel pair uses it
+    BarState* bar = reinterpret_cast<BarState*>(base + NA_BAR_OFF);
+    hipLaunchKernelGGL(k_noalloc_clear, dim3(256), dim3(256), 0, stream, c.dev.gt_key,
+                       c.dev.gt_head, NA_GT_CAP + 1);
+    OpsDev o = make_ops(ops, 0, keys, t->key_bytes, t->is_map ? values : nullptr, t->val_bytes);
+    OutDev d = t->is_map ? make_out(out, nullptr, nullptr)
+                         : make_out(nullptr, nullptr, static_cast<uint8_t*>(out));
+    rc = dispatch(t, [&](auto l) {
+        return Launch<decltype(l)>::batches(c, o, d, offsets, (uint64_t)nbatches, bar, stream);
     });
     if (rc) return rc;
     return end_call(t, stream);
