@@ -179,7 +179,8 @@ int lslab_set_mixed(lslab_table* t, const uint8_t* ops, const void* keys, uint8_
 /* Batched form (new): nbatches mixed-op batches laid end to end in ops/keys/
  * values/out, batch b = [offsets[b], offsets[b+1]) with offsets a DEVICE
  * array of nbatches+1 uint64.  One persistent launch runs them in order with
- * the two-phase engine (batch b sees every effect of batches < b), so the
+ * the two-phase engine (batch b sees every effect of batches < b; per key a
+ * batch's gets come before its mutations), so the
  * call is one graph node when captured.  out as lslab_map_mixed (maps) or
  * bool per op (sets, values ignored). */
 int lslab_exec_batches(lslab_table* t, size_t nbatches, const uint64_t* offsets,
@@ -192,7 +193,7 @@ int lslab_exec_batches(lslab_table* t, size_t nbatches, const uint64_t* offsets,
  * Same arguments, kernels and results as lslab_exec_batches, but the
  * batched kernel's scratch (grid-barrier state, combining table and
  * final-candidate lists) lives in one static device array of the library
- * (about 7 MiB per GPU, held until the process ends) instead of the table's.
+ * (14 MiB per GPU, held until the process ends) instead of the table's.
  * Limits: the table's max_subbatch must be at most
  * LSLAB_NOALLOC_MAX_SUBBATCH ops and its persistent grid (CUs, or max_blocks
  * when set) at most LSLAB_NOALLOC_MAX_BLOCKS blocks; otherwise the call

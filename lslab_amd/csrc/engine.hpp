@@ -524,64 +524,8 @@ __device__ __forceinline__ uint32_t gt_exchange(const TableDev& t, uint64_t key,
 }
 
 // Writes the result of mutation (lo + li) with opcode op, whose serial
-// predecessor in the sub-batch is op (lo + pred - 1), or, for pred == 0, the
-// key's pre-batch state (found, cur).
-// Entry of key in the combining table, or 0xffffffff (a lookup only),
-// probing from the skip-th slot of key's sequence.
-__device__ __forceinline__ uint32_t gt_find(const TableDev& t, uint64_t key, uint32_t skip = 0) {
-    if (key == GT_EMPTY) return t.gt_mask + 1u;
-    uint32_t h = ((uint32_t)mix64(key) + skip) & t.gt_mask;
-    for (uint32_t probe = skip; probe <= t.gt_mask; ++probe) {
-        const uint64_t cur = ld_agent(&t.gt_key[h]);
-        if (cur == key) return h;
-        if (cur == GT_EMPTY) return 0xffffffffu;
-        h = (h + 1u) & t.gt_mask;
-    }
-    return 0xffffffffu;
-}
-
-// The key in the first slot of key's combining-table sequence (loaded ahead,
-// with the key's slab probe).
-__device__ __forceinline__ uint64_t gt_first(const TableDev& t, uint64_t key) {
-    if (key == GT_EMPTY) return GT_EMPTY;
-    return ld_agent(&t.gt_key[(uint32_t)mix64(key) & t.gt_mask]);
-}
-
-// A get in a later batch of a run (several batches whose writes are deferred
-// to one phase W): the key's state after the run's earlier batches is that of
-// its last mutation so far, found through the combining table.  Returns false
-// for a key the run has not mutated (its table state stands).  gk is
-// gt_first(key).  A mutation of the get's own batch may already have
-// exchanged: the get then reads it, which places the get after that mutation
-// in the batch's serial order.
-template <class L>
-__device__ __forceinline__ bool run_state(const TableDev& t, const OpsDev& o, uint64_t lo,
-                                          uint64_t m, uint64_t key, uint64_t gk, bool& found,
-                                          typename L::V& cur) {
-    using V = typename L::V;
-    uint32_t g;
-    if (key == GT_EMPTY) g = t.gt_mask + 1u;
-    else if (gk == key) g = (uint32_t)mix64(key) & t.gt_mask;
-    else if (gk == GT_EMPTY) return false;
-    else g = gt_find(t, key, 1);
-    if (g == 0xffffffffu) return false;
-    const uint32_t p = ld_agent(&t.gt_head[g]);
-    if (p == 0) return false;
-    if (p > m) {
-        __hip_atomic_fetch_or(t.status, ST_CORRUPT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        return false;
-    }
-    const uint64_t pi = lo + p - 1u;
-    if (ld_opcode(o, pi) == OP_PUT) {
-        found = true;
-        cur = L::IS_MAP ? ld_stride<V>(o.vals, o.vstride, pi) : V(0);
-    } else {
-        found = false;
-        cur = V(0);
-    }
-    return true;
-}
-
+// predecessor in the sub-batch (or run) is op (lo + pred - 1), or, for
+// pred == 0, the key's pre-batch state (found, cur).
 template <class L>
 __device__ __forceinline__ void mut_result(const TableDev& t, const OpsDev& o, const OutDev& out,
                                            uint64_t lo, uint64_t m, uint32_t li, uint8_t op,
@@ -628,33 +572,24 @@ __device__ __forceinline__ void mut_result(const TableDev& t, const OpsDev& o, c
 // With block combining (BC; ct is the round's table) every thread of the
 // block calls this once per round (idle groups with act = false), as it
 // contains a block barrier.
-template <class L, int G, int BT, bool BC, bool LOOK = false>
+template <class L, int G, int BT, bool BC>
 __device__ __forceinline__ void phase_r_op(const TableDev& t, const OpsDev& o, const OutDev& out,
                                            uint64_t lo, uint32_t li, uint64_t m, bool act,
                                            uint8_t op, typename L::K key, typename L::V val,
                                            uint64_t s, int slot, CombTab<BT>* ct, RepEnt* rb,
-                                           uint32_t* rcnt, RepEnt* lrep, uint32_t lcap,
-                                           uint64_t gk) {
+                                           uint32_t* rcnt, RepEnt* lrep, uint32_t lcap) {
     using V = typename L::V;
     const int gl = Grp<G>::lane();
     const bool found = act && slot >= 0;
-    // a get of a later batch of a run whose key the run mutated reads the
-    // run's state, not the table's
-    bool gf = false;
-    V gv = V(0);
-    bool gdone = false;
-    if constexpr (LOOK)
-        gdone = act && op == OP_GET && gl == 0 && run_state<L>(t, o, lo, m, (uint64_t)key, gk, gf, gv);
     V cur = V(0);
     if constexpr (L::IS_MAP) {
-        if (found && !gdone) cur = *val_slot<L>(t, s, slot);
+        if (found) cur = *val_slot<L>(t, s, slot);
     }
     if constexpr (BC) {
-        if (act && op == OP_GET && gl == 0)
-            write_result<L>(out, lo + li, gdone ? gf : found, gdone ? gv : cur);
+        if (act && op == OP_GET && gl == 0) write_result<L>(out, lo + li, found, cur);
     } else {  // no block barrier ahead: gets are done
         if (op == OP_GET) {
-            if (gl == 0) write_result<L>(out, lo + li, gdone ? gf : found, gdone ? gv : cur);
+            if (gl == 0) write_result<L>(out, lo + li, found, cur);
             return;
         }
     }
@@ -794,14 +729,11 @@ __device__ __forceinline__ uint32_t* rep_counter() {
 // Chains longer than the head slab continue synchronously (walk_on).  All
 // groups of a block run the same number of rounds (idle ones inactive), as
 // the block-level combining in phase_r_op requires.
-template <class L, int G, int BT, bool BC, bool LOOK = false>
+template <class L, int G, int BT, bool BC>
 __device__ __forceinline__ void phase_r_range(const TableDev& t, const OpsDev& o,
                                               const OutDev& out, uint64_t lo, uint64_t m,
                                               uint64_t gid, uint64_t ng, RepEnt* rb,
-                                              RepEnt* lrep = nullptr, uint32_t lcap = 0,
-                                              uint64_t first = 0) {
-    // Ops [first, m) of the (sub-)batch or run at lo; LOOK: a later batch of a
-    // run, whose gets consult the combining table (run_state).
+                                              RepEnt* lrep = nullptr, uint32_t lcap = 0) {
     using K = typename L::K;
     using V = typename L::V;
     constexpr int S = PW<L, G>::N;
@@ -809,14 +741,14 @@ __device__ __forceinline__ void phase_r_range(const TableDev& t, const OpsDev& o
     CombTab<BT>* tabs = nullptr;
     uint32_t* rcnt = rep_counter();
     if constexpr (BC) {
-        const uint64_t gid0 = first + (uint64_t)blockIdx.x * blockDim.x / G;
+        const uint64_t gid0 = (uint64_t)blockIdx.x * blockDim.x / G;
         if (gid0 >= m) return;  // block-uniform
         rounds = (m - gid0 + ng - 1) / ng;
         tabs = comb_tables<BT>();
     } else {
-        if (first + gid >= m) return;
+        if (gid >= m) return;
     }
-    uint64_t li = first + gid;
+    uint64_t li = gid;
     uint8_t op = OP_GET;
     K key = K(0);
     V val = V(0);
@@ -825,8 +757,6 @@ __device__ __forceinline__ void phase_r_range(const TableDev& t, const OpsDev& o
 #pragma unroll
     for (int e = 0; e < S; ++e) blk[e] = K(0);
     if (!BC || li < m) probe_issue<L, G, false>(t, bucket_of(t, (uint64_t)key), blk);
-    uint64_t gk = GT_EMPTY;  // LOOK: the combining-table slot of a get's key, loaded ahead
-    if (LOOK && (!BC || li < m) && op == OP_GET) gk = gt_first(t, (uint64_t)key);
     uint64_t n1 = li + ng;
     uint8_t op1 = OP_GET;
     K key1 = K(0);
@@ -851,11 +781,8 @@ __device__ __forceinline__ void phase_r_range(const TableDev& t, const OpsDev& o
 #pragma unroll
         for (int e = 0; e < S; ++e) blk1[e] = K(0);
         if (n1 < m) probe_issue<L, G, false>(t, bucket_of(t, (uint64_t)key1), blk1);
-        uint64_t gk1 = GT_EMPTY;
-        if (LOOK && n1 < m && op1 == OP_GET) gk1 = gt_first(t, (uint64_t)key1);
-        phase_r_op<L, G, BT, BC, LOOK>(t, o, out, lo, (uint32_t)li, m, !BC || li < m, op, key, val,
-                                       s, slot, BC ? tabs + (r & 1) : nullptr, rb, rcnt, lrep,
-                                       lcap, gk);
+        phase_r_op<L, G, BT, BC>(t, o, out, lo, (uint32_t)li, m, !BC || li < m, op, key, val, s,
+                                 slot, BC ? tabs + (r & 1) : nullptr, rb, rcnt, lrep, lcap);
         if constexpr (!BC) {
             if (n1 >= m) break;
         }
@@ -863,7 +790,6 @@ __device__ __forceinline__ void phase_r_range(const TableDev& t, const OpsDev& o
         op = op1;
         key = key1;
         val = val1;
-        gk = gk1;
 #pragma unroll
         for (int e = 0; e < S; ++e) blk[e] = blk1[e];
         n1 = n2;
@@ -989,33 +915,17 @@ __device__ __forceinline__ void w_insert(const TableDev& t, typename L::K key, t
     }
 }
 
-// Phase W of one final-mutation candidate: if it is still its key's last
-// mutation (nothing exchanged after it), it writes the key's final state and
-// frees the combining-table entry.
+// Writes key's final state after a sub-batch or run: the last mutation of
+// the key was op (put of val, or remove), and loc is where the key sat in the
+// table before it (slab << 5 | slot, or NOLOC).  All G lanes call it.
 template <class L, int G, bool RC>
-__device__ __forceinline__ void phase_w_ent(const TableDev& t, const OpsDev& o, uint64_t lo,
-                                            const RepEnt& e) {
-    using K = typename L::K;
+__device__ __forceinline__ void w_apply(const TableDev& t, uint8_t op, typename L::K key,
+                                        typename L::V val, uint64_t loc) {
     using V = typename L::V;
     const int gl = Grp<G>::lane();
-    uint8_t op;
-    K key;
-    V val;
-    // The value is loaded whatever the opcode (every op has a value slot), so
-    // the record's loads and the head check share one round trip.
-    const uint64_t i = lo + e.li;
-    op = ld_opcode(o, i);
-    key = ld_stride<K>(o.keys, o.kstride, i);
-    val = (L::IS_MAP && o.vals) ? ld_stride<V>(o.vals, o.vstride, i) : V(0);
-    if (ld_agent(&t.gt_head[e.g]) != e.li + 1u) return;  // a later mutation of this key exists
-    if (gl == 0) {
-        __hip_atomic_store(&t.gt_head[e.g], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (e.g <= t.gt_mask)
-            __hip_atomic_store(&t.gt_key[e.g], GT_EMPTY, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    if (e.loc != NOLOC) {
-        const uint64_t s = e.loc >> 5;
-        const int slot = (int)(e.loc & 31u);
+    if (loc != NOLOC) {
+        const uint64_t s = loc >> 5;
+        const int slot = (int)(loc & 31u);
         uint32_t old = 0;
         if (gl == 0) {
             if (op == OP_PUT) {
@@ -1041,6 +951,30 @@ __device__ __forceinline__ void phase_w_ent(const TableDev& t, const OpsDev& o, 
             w_insert<L, G>(t, key, val);
         }
     }
+}
+
+// Phase W of one final-mutation candidate: if it is still its key's last
+// mutation (nothing exchanged after it), it writes the key's final state and
+// frees the combining-table entry.
+template <class L, int G, bool RC>
+__device__ __forceinline__ void phase_w_ent(const TableDev& t, const OpsDev& o, uint64_t lo,
+                                            const RepEnt& e) {
+    using K = typename L::K;
+    using V = typename L::V;
+    const int gl = Grp<G>::lane();
+    // The value is loaded whatever the opcode (every op has a value slot), so
+    // the record's loads and the head check share one round trip.
+    const uint64_t i = lo + e.li;
+    const uint8_t op = ld_opcode(o, i);
+    const K key = ld_stride<K>(o.keys, o.kstride, i);
+    const V val = (L::IS_MAP && o.vals) ? ld_stride<V>(o.vals, o.vstride, i) : V(0);
+    if (ld_agent(&t.gt_head[e.g]) != e.li + 1u) return;  // a later mutation of this key exists
+    if (gl == 0) {
+        __hip_atomic_store(&t.gt_head[e.g], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (e.g <= t.gt_mask)
+            __hip_atomic_store(&t.gt_key[e.g], GT_EMPTY, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    w_apply<L, G, RC>(t, op, key, val, e.loc);
 }
 
 // Phase W of a block: the cnt final-mutation candidates of its list (the

@@ -10,7 +10,7 @@
 #include <type_traits>
 
 #include "../../include/lslab_c.h"
-#include "engine.hpp"
+#include "runs.hpp"
 
 using namespace lslab_dev;
 
@@ -94,13 +94,14 @@ __global__ __launch_bounds__(256) void k_phase_w(TableDev t, OpsDev o, uint64_t 
 // shared state only through agent-scope loads, or data its own CU wrote,
 // passes acquire = false.  No cache-wide write-back is issued.  Spins are
 // bounded; on timeout or abort the barrier returns false and the caller
-// leaves the kernel.  reset (optional): an LDS word that thread 0 zeroes
-// while the whole block is inside the barrier.
+// leaves the kernel.  reset, reset2 (optional): LDS words that thread 0
+// zeroes while the whole block is inside the barrier.
 __device__ __forceinline__ bool grid_sync(BarState* bar, uint32_t nblocks, bool acquire,
-                                          uint32_t* reset = nullptr) {
+                                          uint32_t* reset = nullptr, uint32_t* reset2 = nullptr) {
     wait_vm();
     __syncthreads();
     if (reset && threadIdx.x == 0) *reset = 0;
+    if (reset2 && threadIdx.x == 0) *reset2 = 0;
     __shared__ uint32_t ok_s;
     if (threadIdx.x == 0) {
         uint32_t ok = 1;
@@ -144,33 +145,30 @@ __device__ __forceinline__ bool grid_sync(BarState* bar, uint32_t nblocks, bool 
 }
 
 // Persistent batched kernel: batches in order.  Consecutive batches that
-// together fit one sub-batch form a run: phase R of each batch in turn, with
-// a grid barrier after each, then one phase W for the whole run.  Mutations of
-// the run chain through one combining table, so a later batch's mutations
-// follow the earlier batches' in their keys' chains, and a later batch's gets
-// read the state the earlier batches left (run_state).  A run saves, per batch
-// after its first, one grid barrier and a phase W.  A batch larger than a
-// sub-batch runs alone, as sub-batches with a phase W each.  A run is also
-// capped so that no group handles more ops than a sub-batch gives it, which
-// bounds every block's list of final-mutation candidates by its fixed region.
-// Only batches of at most RUN_JOIN_MAX ops join a run after its first: every
-// get of a later batch reads its key's combining-table head with an
-// agent-scope load, and under skew a hot key's head is read by all of the
-// batch's gets of that key while the batch's own mutations exchange on it
-// (two 2^20-op Zipf 0.99 batches in one run ran at a third of their speed
-// alone).
-constexpr uint64_t RUN_JOIN_MAX = 1ull << 16;
-
+// together hold at most run_cap ops (and at most RUN_MAX_BATCHES of them) form
+// a run, executed with three grid barriers in all (runs.hpp): pass A orders
+// the run's mutations per (key, batch), pass B resolves every get and every
+// first mutation of a chain against the nearest earlier batch, phase W writes
+// each key's final state once.  A batch that forms no run with its successor
+// runs alone as before: phase R, barrier, phase W, barrier, in sub-batches of
+// submax ops when it is larger.
 template <class L, int G, bool BC, bool RC>
 __global__ __launch_bounds__(PERSIST_THREADS) void k_batches(TableDev t, OpsDev o, OutDev out,
                                                  const uint64_t* offsets, uint64_t nbatches,
-                                                 uint64_t submax, BarState* bar) {
+                                                 uint64_t submax, uint64_t run_cap,
+                                                 BarState* bar) {
     const uint64_t ng = (uint64_t)gridDim.x * blockDim.x / G;
     const uint64_t gid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
-    const uint64_t per_group = (submax + ng - 1) / ng;  // ops a group's list share holds
+    const uint64_t gpb = blockDim.x / G;
     uint32_t* rc = rep_counter();
-    RepEnt* rb = t.rep + block_rep_base_fixed(submax, ng, blockDim.x / G, blockIdx.x);
-    if (threadIdx.x == 0) *rc = 0;
+    uint32_t* cc = clr_counter();
+    RunOffs* ro = run_offs();
+    RepEnt* rb = t.rep + block_rep_base_fixed(submax, ng, gpb, blockIdx.x);
+    uint32_t* clr = t.run_clr + block_rep_base_fixed(run_cap, ng, gpb, blockIdx.x);
+    if (threadIdx.x == 0) {
+        *rc = 0;
+        *cc = 0;
+    }
     if constexpr (BC)
         comb_init<PERSIST_THREADS>();
     else
@@ -184,18 +182,10 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_batches(TableDev t, OpsDev 
     while (b < nbatches) {
         const uint64_t lo = offsets[b];
         // the run [b, e): every block computes the same one
-        uint64_t e = b, load = 0;
-        while (e < nbatches) {
-            const uint64_t mm = offsets[e + 1] - offsets[e];
-            if (mm > submax) break;
-            const uint64_t c = (mm + ng - 1) / ng;
-            if (e > b && (mm > RUN_JOIN_MAX || load + c > per_group ||
-                          offsets[e + 1] - lo > submax))
-                break;
-            load += c;
+        uint64_t e = b;
+        while (e < nbatches && e - b < (uint64_t)RUN_MAX_BATCHES && offsets[e + 1] - lo <= run_cap)
             ++e;
-        }
-        if (e == b) {  // a batch larger than a sub-batch
+        if (e <= b + 1) {  // batch b alone, in sub-batches of at most submax ops
             const uint64_t hi = offsets[b + 1];
             for (uint64_t slo = lo; slo < hi; slo += submax) {
                 const uint64_t m = min(submax, hi - slo);
@@ -205,22 +195,20 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_batches(TableDev t, OpsDev 
                 phase_w_block<L, G, RC>(t, o, slo, rb, *rc, rep_lds(), REP_LDS);
                 if (!grid_sync(bar, gridDim.x, true, rc)) return fail();
             }
-            b = e + 1;
+            b = b + 1;
             continue;
         }
-        for (uint64_t x = b; x < e; ++x) {
-            const uint64_t first = offsets[x] - lo, end = offsets[x + 1] - lo;
-            if (end == first) continue;  // empty batch (block-uniform)
-            if (x == b)  // first == 0
-                phase_r_range<L, G, PERSIST_THREADS, BC, false>(t, o, out, lo, end, gid, ng, rb,
-                                                                rep_lds(), REP_LDS);
-            else
-                phase_r_range<L, G, PERSIST_THREADS, BC, true>(t, o, out, lo, end, gid, ng, rb,
-                                                               rep_lds(), REP_LDS, first);
-            if (!grid_sync(bar, gridDim.x, false)) return fail();
-        }
-        phase_w_block<L, G, RC>(t, o, lo, rb, *rc, rep_lds(), REP_LDS);
-        if (!grid_sync(bar, gridDim.x, true, rc)) return fail();
+        const uint64_t m = offsets[e] - lo;
+        if (threadIdx.x <= e - b) ro->off[threadIdx.x] = offsets[b + threadIdx.x] - lo;
+        if (threadIdx.x == 0) ro->nb = (int)(e - b);
+        __syncthreads();
+        run_pass_a<L, G>(t, o, out, lo, m, *ro, gid, ng, clr, cc);
+        // pass B reads the run tables with plain loads: drop stale lines
+        if (!grid_sync(bar, gridDim.x, true)) return fail();
+        run_pass_b<L, G>(t, o, out, lo, m, *ro, gid, ng, rb, rep_lds(), REP_LDS, rc);
+        if (!grid_sync(bar, gridDim.x, false)) return fail();
+        run_phase_w<L, G, RC>(t, o, lo, rb, *rc, rep_lds(), REP_LDS, clr, *cc);
+        if (!grid_sync(bar, gridDim.x, true, rc, cc)) return fail();
         b = e;
     }
 }
@@ -478,8 +466,13 @@ struct LaunchCtx {
     TableDev dev;
     lslab_tuning tun;  // normalised and checked
     uint64_t submax;   // two-phase ops per sub-batch (scratch capacity)
+    uint64_t run_cap;  // ops a run of batches may hold (run scratch capacity)
     int num_cus;
 };
+
+// Ops a run of the batched kernel may hold: the sub-batch size, at most 2^20
+// (the run tables take 96 B per op of this capacity).
+constexpr uint64_t RUN_CAP_MAX = 1ull << 20;
 
 // Host launchers of one layout; each returns an LSLAB_* status.
 template <class L>

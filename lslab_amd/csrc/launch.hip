@@ -118,7 +118,7 @@ int Launch<L>::batches(const LaunchCtx& c, const OpsDev& o, const OutDev& out,
                 hipLaunchKernelGGL(k_reset_bar<L>, dim3(1), dim3(64), 0, s, bar);
                 hipLaunchKernelGGL((k_batches<L, G, BC, RC>), dim3((unsigned)grid),
                                    dim3(PERSIST_THREADS), 0, s, c.dev, o, out, offsets, nbatches,
-                                   c.submax, bar);
+                                   c.submax, c.run_cap, bar);
                 return launched();
             });
         });

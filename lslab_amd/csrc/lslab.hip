@@ -22,7 +22,7 @@ struct lslab_table {
     lslab_tuning tun;
     TableDev dev;
     DevState* state;
-    uint64_t gt_cap, submax;
+    uint64_t gt_cap, submax, run_cap;
     hipEvent_t order_ev;
     int device;
     int num_cus;
@@ -54,7 +54,13 @@ constexpr uint64_t NA_GTK_OFF = na_align(NA_BAR_OFF + sizeof(BarState));
 constexpr uint64_t NA_GTH_OFF = na_align(NA_GTK_OFF + (NA_GT_CAP + 1) * sizeof(uint64_t));
 constexpr uint64_t NA_REP_OFF = na_align(NA_GTH_OFF + (NA_GT_CAP + 1) * sizeof(uint32_t));
 constexpr uint64_t NA_REP_LEN = NA_SUB + NA_BLOCKS * PERSIST_THREADS;
-constexpr uint64_t NA_BYTES = na_align(NA_REP_OFF + NA_REP_LEN * sizeof(RepEnt));
+// run tables of the batched kernel (runs of at most NA_SUB ops)
+constexpr uint64_t NA_RK_OFF = na_align(NA_REP_OFF + NA_REP_LEN * sizeof(RepEnt));
+constexpr uint64_t NA_RC_OFF = na_align(NA_RK_OFF + (NA_GT_CAP + 2) * sizeof(RunKey));
+constexpr uint64_t NA_RC_LEN = 4 * NA_SUB + RUN_MAX_BATCHES;
+constexpr uint64_t NA_PRED_OFF = na_align(NA_RC_OFF + NA_RC_LEN * sizeof(RunChain));
+constexpr uint64_t NA_CLR_OFF = na_align(NA_PRED_OFF + NA_SUB * sizeof(uint32_t));
+constexpr uint64_t NA_BYTES = na_align(NA_CLR_OFF + NA_REP_LEN * sizeof(uint32_t));
 static_assert(NA_BYTES <= 16ull << 20, "static scratch stays under 16 MiB");
 }  // namespace
 
@@ -82,13 +88,18 @@ static char* noalloc_ptr(int device) {
     return na_ptr[device];
 }
 
-// Empties the static combining table (key = all ones, head = 0) before a
-// call: a kernel, not a memset, so the call can be captured into a graph.
-__global__ void k_noalloc_clear(uint64_t* gt_key, uint32_t* gt_head, uint64_t n) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+// Empties the static combining table and run tables (keys all ones, heads
+// and bitmaps 0) before a call: a kernel, not a memset, so the call can be
+// captured into a graph.
+__global__ void k_noalloc_clear(uint64_t* gt_key, uint32_t* gt_head, RunKey* rk, RunChain* rc) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < NA_RC_LEN;
          i += (uint64_t)gridDim.x * blockDim.x) {
-        gt_key[i] = GT_EMPTY;
-        gt_head[i] = 0;
+        if (i < NA_GT_CAP + 1) {
+            gt_key[i] = GT_EMPTY;
+            gt_head[i] = 0;
+        }
+        if (i < NA_GT_CAP + 2) rk[i] = RunKey{GT_EMPTY, 0};
+        rc[i] = RunChain{GT_EMPTY, 0, 0};
     }
 }
 
@@ -136,6 +147,10 @@ static void free_table(lslab_table* t) {
     (void)hipFree(t->dev.gt_head);
     (void)hipFree(t->dev.rep);
     (void)hipFree(t->dev.rep_cnt);
+    (void)hipFree(t->dev.rk);
+    (void)hipFree(t->dev.rc);
+    (void)hipFree(t->dev.run_pred);
+    (void)hipFree(t->dev.run_clr);
     if (t->order_ev) (void)hipEventDestroy(t->order_ev);
     delete t;
 }
@@ -151,11 +166,35 @@ static uint64_t rep_len(const lslab_table* t) {
     return t->tun.max_subbatch + (uint64_t)t->num_cus * PERSIST_THREADS;
 }
 
+// Run tables of the batched kernel: key table (rk_cap + 2 entries), chain
+// tables (4 * run_cap + RUN_MAX_BATCHES entries), the per-op predecessors
+// (run_cap) and the per-block clear lists (run_cap + one region slack per
+// group of the persistent grid).
+static uint64_t rk_cap_of(uint64_t run_cap) {
+    uint64_t cap = 1;
+    while (cap < 2 * run_cap) cap <<= 1;
+    return cap;
+}
+static uint64_t run_clr_len(const lslab_table* t, uint64_t run_cap) {
+    return run_cap + (uint64_t)t->num_cus * PERSIST_THREADS;
+}
+
 // Device memory the library holds for t (caller-provided slab memory excluded).
 static uint64_t table_device_bytes(const lslab_table* t) {
     return (t->own_slabs ? (t->nbuckets + t->pool) * t->slab_bytes : 0) + t->nbuckets * 4 +
            sizeof(DevState) + (t->gt_cap + 1) * 12 + rep_len(t) * sizeof(RepEnt) +
-           rep_cnt_len(t->submax) * 4;
+           rep_cnt_len(t->submax) * 4 + (rk_cap_of(t->run_cap) + 2) * sizeof(RunKey) +
+           (4 * t->run_cap + RUN_MAX_BATCHES) * sizeof(RunChain) + t->run_cap * 4 +
+           run_clr_len(t, t->run_cap) * 4;
+}
+
+// Empties run tables (setup only; the kernels leave them empty after every run).
+__global__ void k_init_run(RunKey* rk, uint64_t nk, RunChain* rc, uint64_t nc) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nk + nc;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        if (i < nk) rk[i] = RunKey{GT_EMPTY, 0};
+        else rc[i - nk] = RunChain{GT_EMPTY, 0, 0};
+    }
 }
 
 // Sizes the two-phase scratch for t->tun.max_subbatch.  The new buffers are
@@ -164,37 +203,69 @@ static uint64_t table_device_bytes(const lslab_table* t) {
 static int alloc_scratch(lslab_table* t) {
     uint64_t cap = 1;
     while (cap < 2 * t->tun.max_subbatch) cap <<= 1;
+    const uint64_t run_cap = std::min<uint64_t>(t->tun.max_subbatch, RUN_CAP_MAX);
+    const uint64_t rk_cap = rk_cap_of(run_cap);
+    const uint64_t rc_len = 4 * run_cap + RUN_MAX_BATCHES;
     uint64_t* gt_key = nullptr;
     uint32_t* gt_head = nullptr;
     RepEnt* rep = nullptr;
     uint32_t* rep_cnt = nullptr;
-    if (hipMalloc(&gt_key, (cap + 1) * sizeof(uint64_t)) != hipSuccess ||
-        hipMalloc(&gt_head, (cap + 1) * sizeof(uint32_t)) != hipSuccess ||
-        hipMalloc(&rep, rep_len(t) * sizeof(RepEnt)) != hipSuccess ||
-        hipMalloc(&rep_cnt, rep_cnt_len(t->tun.max_subbatch) * sizeof(uint32_t)) != hipSuccess ||
-        hipMemset(gt_key, 0xff, (cap + 1) * sizeof(uint64_t)) != hipSuccess ||
-        hipMemset(gt_head, 0, (cap + 1) * sizeof(uint32_t)) != hipSuccess) {
+    RunKey* rk = nullptr;
+    RunChain* rc = nullptr;
+    uint32_t* run_pred = nullptr;
+    uint32_t* run_clr = nullptr;
+    bool ok = hipMalloc(&gt_key, (cap + 1) * sizeof(uint64_t)) == hipSuccess &&
+              hipMalloc(&gt_head, (cap + 1) * sizeof(uint32_t)) == hipSuccess &&
+              hipMalloc(&rep, rep_len(t) * sizeof(RepEnt)) == hipSuccess &&
+              hipMalloc(&rep_cnt, rep_cnt_len(t->tun.max_subbatch) * sizeof(uint32_t)) ==
+                  hipSuccess &&
+              hipMalloc(&rk, (rk_cap + 2) * sizeof(RunKey)) == hipSuccess &&
+              hipMalloc(&rc, rc_len * sizeof(RunChain)) == hipSuccess &&
+              hipMalloc(&run_pred, run_cap * sizeof(uint32_t)) == hipSuccess &&
+              hipMalloc(&run_clr, run_clr_len(t, run_cap) * sizeof(uint32_t)) == hipSuccess;
+    if (ok) {
+        hipLaunchKernelGGL(k_init_run, dim3(1024), dim3(256), 0, 0, rk, rk_cap + 2, rc, rc_len);
+        ok = hipGetLastError() == hipSuccess &&
+             hipMemset(gt_key, 0xff, (cap + 1) * sizeof(uint64_t)) == hipSuccess &&
+             hipMemset(gt_head, 0, (cap + 1) * sizeof(uint32_t)) == hipSuccess &&
+             hipDeviceSynchronize() == hipSuccess;
+    }
+    if (!ok) {
         (void)hipFree(gt_key);
         (void)hipFree(gt_head);
         (void)hipFree(rep);
         (void)hipFree(rep_cnt);
+        (void)hipFree(rk);
+        (void)hipFree(rc);
+        (void)hipFree(run_pred);
+        (void)hipFree(run_clr);
         return LSLAB_ERR_OOM;
     }
     (void)hipFree(t->dev.gt_key);
     (void)hipFree(t->dev.gt_head);
     (void)hipFree(t->dev.rep);
     (void)hipFree(t->dev.rep_cnt);
+    (void)hipFree(t->dev.rk);
+    (void)hipFree(t->dev.rc);
+    (void)hipFree(t->dev.run_pred);
+    (void)hipFree(t->dev.run_clr);
     t->dev.gt_key = gt_key;
     t->dev.gt_head = gt_head;
     t->dev.rep = rep;
     t->dev.rep_cnt = rep_cnt;
+    t->dev.rk = rk;
+    t->dev.rc = rc;
+    t->dev.run_pred = run_pred;
+    t->dev.run_clr = run_clr;
+    t->dev.rc_cap = 4 * run_cap;
+    t->dev.rk_mask = (uint32_t)(rk_cap - 1);
     t->gt_cap = cap;
     t->submax = t->tun.max_subbatch;
+    t->run_cap = run_cap;
     t->dev.gt_mask = (uint32_t)(cap - 1);
     return LSLAB_OK;
 }
 
-// Device bytes per slab (a u32 set's next link included).
 static uint64_t layout_slab_bytes(TableType type) {
     switch (type) {
         case T_MAP_32_32: return L_M3232::FOOTPRINT;
@@ -423,6 +494,7 @@ static LaunchCtx ctx(const lslab_table* t) {
     c.dev = t->dev;
     c.tun = t->tun;
     c.submax = t->submax;
+    c.run_cap = t->run_cap;
     c.num_cus = t->num_cus;
     return c;
 }
@@ -569,9 +641,15 @@ extern "C" int lslab_exec_batches_noalloc(lslab_table* t, size_t nbatches, const
     c.dev.gt_mask = (uint32_t)(NA_GT_CAP - 1);
     c.dev.rep = reinterpret_cast<RepEnt*>(base + NA_REP_OFF);
     c.dev.rep_cnt = nullptr;  // only the per-call kernel pair uses it
+    c.dev.rk = reinterpret_cast<RunKey*>(base + NA_RK_OFF);
+    c.dev.rk_mask = (uint32_t)(NA_GT_CAP - 1);
+    c.dev.rc = reinterpret_cast<RunChain*>(base + NA_RC_OFF);
+    c.dev.rc_cap = 4 * NA_SUB;
+    c.dev.run_pred = reinterpret_cast<uint32_t*>(base + NA_PRED_OFF);
+    c.dev.run_clr = reinterpret_cast<uint32_t*>(base + NA_CLR_OFF);
     BarState* bar = reinterpret_cast<BarState*>(base + NA_BAR_OFF);
     hipLaunchKernelGGL(k_noalloc_clear, dim3(256), dim3(256), 0, stream, c.dev.gt_key,
-                       c.dev.gt_head, NA_GT_CAP + 1);
+                       c.dev.gt_head, c.dev.rk, c.dev.rc);
     OpsDev o = make_ops(ops, 0, keys, t->key_bytes, t->is_map ? values : nullptr, t->val_bytes);
     OutDev d = t->is_map ? make_out(out, nullptr, nullptr)
                          : make_out(nullptr, nullptr, static_cast<uint8_t*>(out));

@@ -71,6 +71,21 @@ struct RepEnt {
     uint64_t loc;
 };
 
+// Scratch of a run of batches in the persistent batched kernel.  The key
+// table holds, per key the run mutates, the bitmap of the run's batches that
+// mutate it; the chain table holds, per (key, batch), the last mutation of
+// that batch's exchange chain.  Key and payload share one 16-B entry, so one
+// load reads both.
+struct RunKey {
+    uint64_t key;   // GT_EMPTY = free
+    uint64_t mask;  // bit r: batch r of the run mutates the key
+};
+struct RunChain {
+    uint64_t key;   // GT_EMPTY = free
+    uint32_t head;  // last exchanged mutation (run op index + 1)
+    uint32_t pad;
+};
+
 struct TableDev {
     char* slabs;           // (nbuckets + pool) * SLAB_BYTES, 256-B aligned, then
                            // (u32 sets) one u32 next link per slab
@@ -88,6 +103,14 @@ struct TableDev {
     uint32_t hash_mix;     // bucket by mix64(key) instead of the key itself
     uint64_t* free_head;   // reclaim: free pool slabs, (tag << 32 | slab), 0 = empty
     uint64_t* free_count;  // reclaim: slabs on the free list
+    RunKey* rk;            // run key table: rk_mask + 2 entries (the last: the all-ones key)
+    RunChain* rc;          // run chain tables: rc_cap entries, then one per batch of a
+                           // run for the all-ones key
+    uint32_t* run_pred;    // per op of a run: chain predecessor (op index + 1) or
+                           // RUN_CF | chain entry for the first mutation of a chain
+    uint32_t* run_clr;     // chain entries to clear, one list per block (as rep)
+    uint64_t rc_cap;       // 4 * (ops a run may hold)
+    uint32_t rk_mask;      // key-table capacity - 1 (at least 2 * ops a run may hold)
 };
 
 // Operation stream: per-op opcode (or a uniform one), keys and values with a

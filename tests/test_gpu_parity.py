@@ -603,10 +603,12 @@ def test_caller_slab_memory(lslab, oracle_mod, is_map, kb, vb, engine):
 
 @pytest.mark.parametrize("is_map", [True, False])
 def test_exec_batches_run_boundaries(lslab, oracle_mod, is_map):
-    """Batches around the run limits: a batch of exactly 2^16 ops may join a run,
-    one op more starts its own; a batch larger than the sub-batch runs alone
-    in sub-batches; empty batches inside runs.  Each batch is checked in order
-    against the oracle's serial-order checker, then the final contents."""
+    """Batches around the run limits (runs hold at most the sub-batch size,
+    2^18 ops here): batches of 2^16 ops that fill runs exactly, a batch that
+    cannot join the run before it, a batch larger than the sub-batch (runs
+    alone in sub-batches), empty batches inside runs.  Each batch is checked
+    in order against the oracle's serial-order checker, then the final
+    contents."""
     say("exec_batches_run_boundaries", is_map)
     rng = np.random.default_rng(41)
     kb, vb = 8, 8
