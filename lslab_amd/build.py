@@ -20,7 +20,7 @@ LAYOUTS = ["L_M3232", "L_M3264", "L_M6432", "L_M6464", "L_S32", "L_S64"]
 UNITS = ([("lslab", "csrc/lslab.hip", []), ("workload", "csrc/workload.hip", []),
           ("kvcg", "csrc/kvcg.cpp", []), ("wide", "csrc/wide.hip", [])] +
          [(f"launch_{l.lower()}", "csrc/launch.hip", [f"-DLSLAB_LAYOUT={l}"]) for l in LAYOUTS])
-HEADERS = ["csrc/slab.hpp", "csrc/engine.hpp", "csrc/kernels.hpp"]
+HEADERS = ["csrc/slab.hpp", "csrc/engine.hpp", "csrc/runs.hpp", "csrc/kernels.hpp"]
 OUT = os.path.join(HERE, "liblslab.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

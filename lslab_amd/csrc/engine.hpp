@@ -538,15 +538,13 @@ __device__ __forceinline__ void mut_result(const TableDev& t, const OpsDev& o, c
         pred = 0;
     }
     if (pred != 0) {
+        // The predecessor's opcode and value are loaded together (one round
+        // trip); the value counts only for a put.
         const uint64_t pi = lo + pred - 1u;
         const uint8_t pop = ld_opcode(o, pi);
-        if (pop == OP_PUT) {
-            rf = true;
-            rv = L::IS_MAP ? ld_stride<V>(o.vals, o.vstride, pi) : V(0);
-        } else {
-            rf = false;
-            rv = V(0);
-        }
+        const V pv = (L::IS_MAP && o.vals) ? ld_stride<V>(o.vals, o.vstride, pi) : V(0);
+        rf = pop == OP_PUT;
+        rv = rf ? pv : V(0);
     }
     if (!L::IS_MAP && op == OP_PUT) rf = true;  // set insert reports true
     write_result<L>(out, lo + li, rf, rv);

@@ -146,12 +146,16 @@ __device__ __forceinline__ bool grid_sync(BarState* bar, uint32_t nblocks, bool 
 
 // Persistent batched kernel: batches in order.  Consecutive batches that
 // together hold at most run_cap ops (and at most RUN_MAX_BATCHES of them) form
-// a run, executed with three grid barriers in all (runs.hpp): pass A orders
-// the run's mutations per (key, batch), pass B resolves every get and every
-// first mutation of a chain against the nearest earlier batch, phase W writes
-// each key's final state once.  A batch that forms no run with its successor
-// runs alone as before: phase R, barrier, phase W, barrier, in sub-batches of
-// submax ops when it is larger.
+// a run (runs.hpp): pass A orders the run's mutations per (key, batch), pass
+// B resolves every get and every first mutation of a chain against the
+// nearest earlier batch, phase W writes each key's final state once.  Runs
+// alternate between two sets of run tables, so phase W of a run shares its
+// step with pass A of the next run (pass A reads nothing of the table and
+// writes only its own set): two grid barriers per run,
+//     [W(k-1) | A(k)]  barrier+acquire  B(k)  barrier  [W(k) | A(k+1)] ...
+// A batch that forms no run with its successor runs alone after any pending
+// phase W: phase R, barrier, phase W, barrier, in sub-batches of submax ops
+// when it is larger.
 template <class L, int G, bool BC, bool RC>
 __global__ __launch_bounds__(PERSIST_THREADS) void k_batches(TableDev t, OpsDev o, OutDev out,
                                                  const uint64_t* offsets, uint64_t nbatches,
@@ -161,13 +165,14 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_batches(TableDev t, OpsDev 
     const uint64_t gid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
     const uint64_t gpb = blockDim.x / G;
     uint32_t* rc = rep_counter();
-    uint32_t* cc = clr_counter();
+    uint32_t* cc = clr_counters();
     RunOffs* ro = run_offs();
     RepEnt* rb = t.rep + block_rep_base_fixed(submax, ng, gpb, blockIdx.x);
-    uint32_t* clr = t.run_clr + block_rep_base_fixed(run_cap, ng, gpb, blockIdx.x);
+    const uint64_t clr_base = block_rep_base_fixed(run_cap, ng, gpb, blockIdx.x);
     if (threadIdx.x == 0) {
         *rc = 0;
-        *cc = 0;
+        cc[0] = 0;
+        cc[1] = 0;
     }
     if constexpr (BC)
         comb_init<PERSIST_THREADS>();
@@ -178,15 +183,33 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_batches(TableDev t, OpsDev 
             __hip_atomic_fetch_or(t.status, ST_TIMEOUT, __ATOMIC_RELAXED,
                                   __HIP_MEMORY_SCOPE_AGENT);
     };
+    int par = 0;         // table set of the next run
+    bool pend = false;   // the previous run's phase W is still to do (set 1 - par)
+    uint64_t plo = 0;    // that run's first op
     uint64_t b = 0;
     while (b < nbatches) {
-        const uint64_t lo = offsets[b];
-        // the run [b, e): every block computes the same one
-        uint64_t e = b;
-        while (e < nbatches && e - b < (uint64_t)RUN_MAX_BATCHES && offsets[e + 1] - lo <= run_cap)
-            ++e;
-        if (e <= b + 1) {  // batch b alone, in sub-batches of at most submax ops
-            const uint64_t hi = offsets[b + 1];
+        // The run [b, b + nb): one thread per boundary loads the window
+        // offsets[b .. b + RUN_MAX_BATCHES] into LDS, one thread scans it.
+        const uint64_t wn = min((uint64_t)RUN_MAX_BATCHES, nbatches - b);
+        if (threadIdx.x <= wn) ro->win[threadIdx.x] = offsets[b + threadIdx.x];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint64_t e = 0;
+            while (e < wn && ro->win[e + 1] - ro->win[0] <= run_cap) ++e;
+            ro->nb = (int)e;
+        }
+        __syncthreads();
+        const uint64_t lo = ro->win[0];
+        const uint64_t nr = (uint64_t)ro->nb;
+        if (nr <= 1) {  // batch b alone, in sub-batches of at most submax ops
+            const uint64_t hi = ro->win[1];
+            if (pend) {
+                const RunTabs pt = par ? t.run[0] : t.run[1];
+                run_phase_w<L, G, RC>(t, pt, o, plo, rb, *rc, rep_lds(), REP_LDS,
+                                      pt.clr + clr_base, cc[par ^ 1]);
+                if (!grid_sync(bar, gridDim.x, true, rc, &cc[par ^ 1])) return fail();
+                pend = false;
+            }
             for (uint64_t slo = lo; slo < hi; slo += submax) {
                 const uint64_t m = min(submax, hi - slo);
                 phase_r_range<L, G, PERSIST_THREADS, BC>(t, o, out, slo, m, gid, ng, rb,
@@ -195,21 +218,35 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_batches(TableDev t, OpsDev 
                 phase_w_block<L, G, RC>(t, o, slo, rb, *rc, rep_lds(), REP_LDS);
                 if (!grid_sync(bar, gridDim.x, true, rc)) return fail();
             }
+            __syncthreads();  // every wave has read the window before it is reloaded
             b = b + 1;
             continue;
         }
-        const uint64_t m = offsets[e] - lo;
-        if (threadIdx.x <= e - b) ro->off[threadIdx.x] = offsets[b + threadIdx.x] - lo;
-        if (threadIdx.x == 0) ro->nb = (int)(e - b);
+        const uint64_t m = ro->win[nr] - lo;
+        if (threadIdx.x <= nr) ro->off[threadIdx.x] = ro->win[threadIdx.x] - lo;
         __syncthreads();
-        run_pass_a<L, G>(t, o, out, lo, m, *ro, gid, ng, clr, cc);
-        // pass B reads the run tables with plain loads: drop stale lines
-        if (!grid_sync(bar, gridDim.x, true)) return fail();
-        run_pass_b<L, G>(t, o, out, lo, m, *ro, gid, ng, rb, rep_lds(), REP_LDS, rc);
+        const RunTabs rt = par ? t.run[1] : t.run[0];
+        if (pend) {  // the previous run's write-back, in the other table set
+            const RunTabs pt = par ? t.run[0] : t.run[1];
+            run_phase_w<L, G, RC>(t, pt, o, plo, rb, *rc, rep_lds(), REP_LDS, pt.clr + clr_base,
+                                  cc[par ^ 1]);
+        }
+        run_pass_a<L, G>(t, rt, o, out, lo, m, *ro, gid, ng, rt.clr + clr_base, &cc[par]);
+        // pass B reads the table (after that write-back) and the run tables
+        // with plain loads: drop stale lines
+        if (!grid_sync(bar, gridDim.x, true, rc, &cc[par ^ 1])) return fail();
+        run_pass_b<L, G>(t, rt, o, out, lo, m, *ro, gid, ng, rb, rep_lds(), REP_LDS, rc);
         if (!grid_sync(bar, gridDim.x, false)) return fail();
-        run_phase_w<L, G, RC>(t, o, lo, rb, *rc, rep_lds(), REP_LDS, clr, *cc);
-        if (!grid_sync(bar, gridDim.x, true, rc, cc)) return fail();
-        b = e;
+        pend = true;
+        plo = lo;
+        par ^= 1;
+        b += nr;
+    }
+    if (pend) {
+        const RunTabs pt = par ? t.run[0] : t.run[1];
+        run_phase_w<L, G, RC>(t, pt, o, plo, rb, *rc, rep_lds(), REP_LDS, pt.clr + clr_base,
+                              cc[par ^ 1]);
+        if (!grid_sync(bar, gridDim.x, true, rc, &cc[par ^ 1])) return fail();
     }
 }
 

@@ -54,14 +54,16 @@ constexpr uint64_t NA_GTK_OFF = na_align(NA_BAR_OFF + sizeof(BarState));
 constexpr uint64_t NA_GTH_OFF = na_align(NA_GTK_OFF + (NA_GT_CAP + 1) * sizeof(uint64_t));
 constexpr uint64_t NA_REP_OFF = na_align(NA_GTH_OFF + (NA_GT_CAP + 1) * sizeof(uint32_t));
 constexpr uint64_t NA_REP_LEN = NA_SUB + NA_BLOCKS * PERSIST_THREADS;
-// run tables of the batched kernel (runs of at most NA_SUB ops)
-constexpr uint64_t NA_RK_OFF = na_align(NA_REP_OFF + NA_REP_LEN * sizeof(RepEnt));
-constexpr uint64_t NA_RC_OFF = na_align(NA_RK_OFF + (NA_GT_CAP + 2) * sizeof(RunKey));
+// two sets of run tables of the batched kernel (runs of at most NA_SUB ops)
+constexpr uint64_t NA_RK_LEN = NA_GT_CAP + 2;
 constexpr uint64_t NA_RC_LEN = 4 * NA_SUB + RUN_MAX_BATCHES;
-constexpr uint64_t NA_PRED_OFF = na_align(NA_RC_OFF + NA_RC_LEN * sizeof(RunChain));
-constexpr uint64_t NA_CLR_OFF = na_align(NA_PRED_OFF + NA_SUB * sizeof(uint32_t));
-constexpr uint64_t NA_BYTES = na_align(NA_CLR_OFF + NA_REP_LEN * sizeof(uint32_t));
-static_assert(NA_BYTES <= 16ull << 20, "static scratch stays under 16 MiB");
+constexpr uint64_t NA_SET_BYTES = na_align(NA_RK_LEN * sizeof(RunKey)) +
+                                  na_align(NA_RC_LEN * sizeof(RunChain)) +
+                                  na_align(NA_REP_LEN * sizeof(uint32_t));
+constexpr uint64_t NA_SET_OFF = na_align(NA_REP_OFF + NA_REP_LEN * sizeof(RepEnt));
+constexpr uint64_t NA_PRED_OFF = NA_SET_OFF + 2 * NA_SET_BYTES;
+constexpr uint64_t NA_BYTES = na_align(NA_PRED_OFF + NA_SUB * sizeof(uint32_t));
+static_assert(NA_BYTES <= 24ull << 20, "static scratch stays under 24 MiB");
 }  // namespace
 
 static __device__ __align__(256) unsigned char scratch[NA_BYTES];
@@ -88,19 +90,30 @@ static char* noalloc_ptr(int device) {
     return na_ptr[device];
 }
 
-// Empties the static combining table and run tables (keys all ones, heads
-// and bitmaps 0) before a call: a kernel, not a memset, so the call can be
-// captured into a graph.
+// Empties the static combining table and one set of run tables (keys all
+// ones, heads and bitmaps 0) before a call: a kernel, not a memset, so the
+// call can be captured into a graph.
 __global__ void k_noalloc_clear(uint64_t* gt_key, uint32_t* gt_head, RunKey* rk, RunChain* rc) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < NA_RC_LEN;
          i += (uint64_t)gridDim.x * blockDim.x) {
-        if (i < NA_GT_CAP + 1) {
+        if (gt_key && i < NA_GT_CAP + 1) {
             gt_key[i] = GT_EMPTY;
             gt_head[i] = 0;
         }
-        if (i < NA_GT_CAP + 2) rk[i] = RunKey{GT_EMPTY, 0};
+        if (i < NA_RK_LEN) rk[i] = RunKey{GT_EMPTY, 0};
         rc[i] = RunChain{GT_EMPTY, 0, 0};
     }
+}
+
+// Set p of the static run tables at base.
+static RunTabs noalloc_set(char* base, int p) {
+    char* s = base + NA_SET_OFF + (uint64_t)p * NA_SET_BYTES;
+    RunTabs r;
+    r.rk = reinterpret_cast<RunKey*>(s);
+    r.rc = reinterpret_cast<RunChain*>(s + na_align(NA_RK_LEN * sizeof(RunKey)));
+    r.clr = reinterpret_cast<uint32_t*>(s + na_align(NA_RK_LEN * sizeof(RunKey)) +
+                                        na_align(NA_RC_LEN * sizeof(RunChain)));
+    return r;
 }
 
 template <typename F>
@@ -147,10 +160,12 @@ static void free_table(lslab_table* t) {
     (void)hipFree(t->dev.gt_head);
     (void)hipFree(t->dev.rep);
     (void)hipFree(t->dev.rep_cnt);
-    (void)hipFree(t->dev.rk);
-    (void)hipFree(t->dev.rc);
+    for (RunTabs& r : t->dev.run) {
+        (void)hipFree(r.rk);
+        (void)hipFree(r.rc);
+        (void)hipFree(r.clr);
+    }
     (void)hipFree(t->dev.run_pred);
-    (void)hipFree(t->dev.run_clr);
     if (t->order_ev) (void)hipEventDestroy(t->order_ev);
     delete t;
 }
@@ -166,10 +181,10 @@ static uint64_t rep_len(const lslab_table* t) {
     return t->tun.max_subbatch + (uint64_t)t->num_cus * PERSIST_THREADS;
 }
 
-// Run tables of the batched kernel: key table (rk_cap + 2 entries), chain
-// tables (4 * run_cap + RUN_MAX_BATCHES entries), the per-op predecessors
-// (run_cap) and the per-block clear lists (run_cap + one region slack per
-// group of the persistent grid).
+// Run tables of the batched kernel, two sets: key table (rk_cap + 2 entries),
+// chain tables (4 * run_cap + RUN_MAX_BATCHES entries) and per-block clear
+// lists (run_cap + one region slack per group of the persistent grid); and
+// the per-op predecessors (run_cap).
 static uint64_t rk_cap_of(uint64_t run_cap) {
     uint64_t cap = 1;
     while (cap < 2 * run_cap) cap <<= 1;
@@ -183,9 +198,10 @@ static uint64_t run_clr_len(const lslab_table* t, uint64_t run_cap) {
 static uint64_t table_device_bytes(const lslab_table* t) {
     return (t->own_slabs ? (t->nbuckets + t->pool) * t->slab_bytes : 0) + t->nbuckets * 4 +
            sizeof(DevState) + (t->gt_cap + 1) * 12 + rep_len(t) * sizeof(RepEnt) +
-           rep_cnt_len(t->submax) * 4 + (rk_cap_of(t->run_cap) + 2) * sizeof(RunKey) +
-           (4 * t->run_cap + RUN_MAX_BATCHES) * sizeof(RunChain) + t->run_cap * 4 +
-           run_clr_len(t, t->run_cap) * 4;
+           rep_cnt_len(t->submax) * 4 + t->run_cap * 4 +
+           2 * ((rk_cap_of(t->run_cap) + 2) * sizeof(RunKey) +
+                (4 * t->run_cap + RUN_MAX_BATCHES) * sizeof(RunChain) +
+                run_clr_len(t, t->run_cap) * 4);
 }
 
 // Empties run tables (setup only; the kernels leave them empty after every run).
@@ -210,53 +226,53 @@ static int alloc_scratch(lslab_table* t) {
     uint32_t* gt_head = nullptr;
     RepEnt* rep = nullptr;
     uint32_t* rep_cnt = nullptr;
-    RunKey* rk = nullptr;
-    RunChain* rc = nullptr;
+    RunTabs run[2] = {};
     uint32_t* run_pred = nullptr;
-    uint32_t* run_clr = nullptr;
     bool ok = hipMalloc(&gt_key, (cap + 1) * sizeof(uint64_t)) == hipSuccess &&
               hipMalloc(&gt_head, (cap + 1) * sizeof(uint32_t)) == hipSuccess &&
               hipMalloc(&rep, rep_len(t) * sizeof(RepEnt)) == hipSuccess &&
               hipMalloc(&rep_cnt, rep_cnt_len(t->tun.max_subbatch) * sizeof(uint32_t)) ==
                   hipSuccess &&
-              hipMalloc(&rk, (rk_cap + 2) * sizeof(RunKey)) == hipSuccess &&
-              hipMalloc(&rc, rc_len * sizeof(RunChain)) == hipSuccess &&
-              hipMalloc(&run_pred, run_cap * sizeof(uint32_t)) == hipSuccess &&
-              hipMalloc(&run_clr, run_clr_len(t, run_cap) * sizeof(uint32_t)) == hipSuccess;
-    if (ok) {
-        hipLaunchKernelGGL(k_init_run, dim3(1024), dim3(256), 0, 0, rk, rk_cap + 2, rc, rc_len);
-        ok = hipGetLastError() == hipSuccess &&
-             hipMemset(gt_key, 0xff, (cap + 1) * sizeof(uint64_t)) == hipSuccess &&
-             hipMemset(gt_head, 0, (cap + 1) * sizeof(uint32_t)) == hipSuccess &&
-             hipDeviceSynchronize() == hipSuccess;
+              hipMalloc(&run_pred, run_cap * sizeof(uint32_t)) == hipSuccess;
+    for (RunTabs& r : run) {
+        ok = ok && hipMalloc(&r.rk, (rk_cap + 2) * sizeof(RunKey)) == hipSuccess &&
+             hipMalloc(&r.rc, rc_len * sizeof(RunChain)) == hipSuccess &&
+             hipMalloc(&r.clr, run_clr_len(t, run_cap) * sizeof(uint32_t)) == hipSuccess;
+        if (ok) {
+            hipLaunchKernelGGL(k_init_run, dim3(1024), dim3(256), 0, 0, r.rk, rk_cap + 2, r.rc,
+                               rc_len);
+            ok = hipGetLastError() == hipSuccess;
+        }
     }
+    ok = ok && hipMemset(gt_key, 0xff, (cap + 1) * sizeof(uint64_t)) == hipSuccess &&
+         hipMemset(gt_head, 0, (cap + 1) * sizeof(uint32_t)) == hipSuccess &&
+         hipDeviceSynchronize() == hipSuccess;
+    auto release = [](uint64_t* gk, uint32_t* gh, RepEnt* rp, uint32_t* rcn, RunTabs* rn,
+                      uint32_t* rpd) {
+        (void)hipFree(gk);
+        (void)hipFree(gh);
+        (void)hipFree(rp);
+        (void)hipFree(rcn);
+        for (int i = 0; i < 2; ++i) {
+            (void)hipFree(rn[i].rk);
+            (void)hipFree(rn[i].rc);
+            (void)hipFree(rn[i].clr);
+        }
+        (void)hipFree(rpd);
+    };
     if (!ok) {
-        (void)hipFree(gt_key);
-        (void)hipFree(gt_head);
-        (void)hipFree(rep);
-        (void)hipFree(rep_cnt);
-        (void)hipFree(rk);
-        (void)hipFree(rc);
-        (void)hipFree(run_pred);
-        (void)hipFree(run_clr);
+        release(gt_key, gt_head, rep, rep_cnt, run, run_pred);
         return LSLAB_ERR_OOM;
     }
-    (void)hipFree(t->dev.gt_key);
-    (void)hipFree(t->dev.gt_head);
-    (void)hipFree(t->dev.rep);
-    (void)hipFree(t->dev.rep_cnt);
-    (void)hipFree(t->dev.rk);
-    (void)hipFree(t->dev.rc);
-    (void)hipFree(t->dev.run_pred);
-    (void)hipFree(t->dev.run_clr);
+    release(t->dev.gt_key, t->dev.gt_head, t->dev.rep, t->dev.rep_cnt, t->dev.run,
+            t->dev.run_pred);
     t->dev.gt_key = gt_key;
     t->dev.gt_head = gt_head;
     t->dev.rep = rep;
     t->dev.rep_cnt = rep_cnt;
-    t->dev.rk = rk;
-    t->dev.rc = rc;
+    t->dev.run[0] = run[0];
+    t->dev.run[1] = run[1];
     t->dev.run_pred = run_pred;
-    t->dev.run_clr = run_clr;
     t->dev.rc_cap = 4 * run_cap;
     t->dev.rk_mask = (uint32_t)(rk_cap - 1);
     t->gt_cap = cap;
@@ -641,15 +657,16 @@ extern "C" int lslab_exec_batches_noalloc(lslab_table* t, size_t nbatches, const
     c.dev.gt_mask = (uint32_t)(NA_GT_CAP - 1);
     c.dev.rep = reinterpret_cast<RepEnt*>(base + NA_REP_OFF);
     c.dev.rep_cnt = nullptr;  // only the per-call kernel pair uses it
-    c.dev.rk = reinterpret_cast<RunKey*>(base + NA_RK_OFF);
+    c.dev.run[0] = noalloc_set(base, 0);
+    c.dev.run[1] = noalloc_set(base, 1);
     c.dev.rk_mask = (uint32_t)(NA_GT_CAP - 1);
-    c.dev.rc = reinterpret_cast<RunChain*>(base + NA_RC_OFF);
     c.dev.rc_cap = 4 * NA_SUB;
     c.dev.run_pred = reinterpret_cast<uint32_t*>(base + NA_PRED_OFF);
-    c.dev.run_clr = reinterpret_cast<uint32_t*>(base + NA_CLR_OFF);
     BarState* bar = reinterpret_cast<BarState*>(base + NA_BAR_OFF);
     hipLaunchKernelGGL(k_noalloc_clear, dim3(256), dim3(256), 0, stream, c.dev.gt_key,
-                       c.dev.gt_head, c.dev.rk, c.dev.rc);
+                       c.dev.gt_head, c.dev.run[0].rk, c.dev.run[0].rc);
+    hipLaunchKernelGGL(k_noalloc_clear, dim3(256), dim3(256), 0, stream, nullptr, nullptr,
+                       c.dev.run[1].rk, c.dev.run[1].rc);
     OpsDev o = make_ops(ops, 0, keys, t->key_bytes, t->is_map ? values : nullptr, t->val_bytes);
     OutDev d = t->is_map ? make_out(out, nullptr, nullptr)
                          : make_out(nullptr, nullptr, static_cast<uint8_t*>(out));

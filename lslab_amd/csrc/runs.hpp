@@ -32,9 +32,11 @@ constexpr int RUN_MAX_BATCHES = 64;    // one bit per batch in a key's bitmap
 constexpr uint32_t RUN_CF = 0x80000000u;  // run_pred: first mutation of its chain
 constexpr uint32_t NOENT = 0xffffffffu;
 
-// The current run's batch boundaries relative to its first op (LDS).
+// The current run's batch boundaries relative to its first op, and the
+// window of absolute boundaries it was cut from (LDS).
 struct RunOffs {
     uint64_t off[RUN_MAX_BATCHES + 1];
+    uint64_t win[RUN_MAX_BATCHES + 1];
     int nb;
 };
 
@@ -43,10 +45,10 @@ __device__ __forceinline__ RunOffs* run_offs() {
     return &ro;
 }
 
-// The block's LDS counter of chain entries to clear.
-__device__ __forceinline__ uint32_t* clr_counter() {
-    __shared__ uint32_t cnt;
-    return &cnt;
+// The block's LDS counters of chain entries to clear, one per table set.
+__device__ __forceinline__ uint32_t* clr_counters() {
+    __shared__ uint32_t cnt[2];
+    return cnt;
 }
 
 // Batch of run op li: the largest r with off[r] <= li (an empty batch owns
@@ -73,17 +75,22 @@ __device__ __forceinline__ void run_region(const RunOffs& ro, int r, uint64_t& b
 }
 
 // Claims or finds key's entry in region r (one compare-and-swap per probe);
-// the all-ones key has a dedicated entry per batch after the regions.
-__device__ __forceinline__ uint32_t rc_claim(const TableDev& t, uint64_t key, uint64_t base,
-                                             uint32_t cmask, int r) {
+// won: this call claimed it (exactly one call per (key, batch) does).  The
+// all-ones key has a dedicated entry per batch after the regions, whose
+// claim is the exchange that finds its head empty.
+__device__ __forceinline__ uint32_t rc_claim(const TableDev& t, const RunTabs& rt, uint64_t key, uint64_t base,
+                                             uint32_t cmask, int r, bool& won) {
+    won = false;
     if (key == GT_EMPTY) return (uint32_t)(t.rc_cap + r);
     uint32_t h = (uint32_t)mix64(key) & cmask;
     for (uint32_t p = 0; p <= cmask; ++p) {
         uint64_t e = GT_EMPTY;
-        if (__hip_atomic_compare_exchange_strong(&t.rc[base + h].key, &e, key, __ATOMIC_RELAXED,
-                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
-            e == key)
+        if (__hip_atomic_compare_exchange_strong(&rt.rc[base + h].key, &e, key, __ATOMIC_RELAXED,
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            won = true;
             return (uint32_t)(base + h);
+        }
+        if (e == key) return (uint32_t)(base + h);
         h = (h + 1u) & cmask;
     }
     return NOENT;  // unreachable: a region is at most half full
@@ -91,12 +98,12 @@ __device__ __forceinline__ uint32_t rc_claim(const TableDev& t, uint64_t key, ui
 
 // Tail (last mutation + 1) of key's chain in region r, read with plain loads
 // (pass B: the table is read-only); 0 if the batch has none.
-__device__ __forceinline__ uint32_t rc_tail(const TableDev& t, uint64_t key, uint64_t base,
+__device__ __forceinline__ uint32_t rc_tail(const TableDev& t, const RunTabs& rt, uint64_t key, uint64_t base,
                                             uint32_t cmask, int r) {
-    if (key == GT_EMPTY) return t.rc[t.rc_cap + r].head;
+    if (key == GT_EMPTY) return rt.rc[t.rc_cap + r].head;
     uint32_t h = (uint32_t)mix64(key) & cmask;
     for (uint32_t p = 0; p <= cmask; ++p) {
-        const uint4 w = *reinterpret_cast<const uint4*>(&t.rc[base + h]);
+        const uint4 w = *reinterpret_cast<const uint4*>(&rt.rc[base + h]);
         const uint64_t k = ((uint64_t)w.y << 32) | w.x;
         if (k == key) return w.z;
         if (k == GT_EMPTY) return 0;
@@ -106,12 +113,12 @@ __device__ __forceinline__ uint32_t rc_tail(const TableDev& t, uint64_t key, uin
 }
 
 // Claims or finds key's entry in the key table (the all-ones key: the last).
-__device__ __forceinline__ uint32_t rk_claim(const TableDev& t, uint64_t key) {
+__device__ __forceinline__ uint32_t rk_claim(const TableDev& t, const RunTabs& rt, uint64_t key) {
     if (key == GT_EMPTY) return t.rk_mask + 1u;
     uint32_t h = (uint32_t)mix64(key) & t.rk_mask;
     for (uint32_t p = 0; p <= t.rk_mask; ++p) {
         uint64_t e = GT_EMPTY;
-        if (__hip_atomic_compare_exchange_strong(&t.rk[h].key, &e, key, __ATOMIC_RELAXED,
+        if (__hip_atomic_compare_exchange_strong(&rt.rk[h].key, &e, key, __ATOMIC_RELAXED,
                                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
             e == key)
             return h;
@@ -123,11 +130,11 @@ __device__ __forceinline__ uint32_t rk_claim(const TableDev& t, uint64_t key) {
 // Key's entry and batch bitmap in the key table, plain loads (pass B).  The
 // first slot's 16-B entry was loaded ahead (w0); NOENT when the run does not
 // mutate the key.
-__device__ __forceinline__ uint32_t rk_lookup(const TableDev& t, uint64_t key, uint4 w0,
+__device__ __forceinline__ uint32_t rk_lookup(const TableDev& t, const RunTabs& rt, uint64_t key, uint4 w0,
                                               uint64_t& mask) {
     mask = 0;
     if (key == GT_EMPTY) {
-        mask = t.rk[t.rk_mask + 1u].mask;
+        mask = rt.rk[t.rk_mask + 1u].mask;
         return mask ? t.rk_mask + 1u : NOENT;
     }
     uint32_t h = (uint32_t)mix64(key) & t.rk_mask;
@@ -140,14 +147,14 @@ __device__ __forceinline__ uint32_t rk_lookup(const TableDev& t, uint64_t key, u
         }
         if (k == GT_EMPTY) return NOENT;
         h = (h + 1u) & t.rk_mask;
-        w = *reinterpret_cast<const uint4*>(&t.rk[h]);
+        w = *reinterpret_cast<const uint4*>(&rt.rk[h]);
     }
     return NOENT;
 }
 
-__device__ __forceinline__ uint4 rk_first(const TableDev& t, uint64_t key) {
+__device__ __forceinline__ uint4 rk_first(const TableDev& t, const RunTabs& rt, uint64_t key) {
     if (key == GT_EMPTY) return make_uint4(0xffffffffu, 0xffffffffu, 0u, 0u);
-    return *reinterpret_cast<const uint4*>(&t.rk[(uint32_t)mix64(key) & t.rk_mask]);
+    return *reinterpret_cast<const uint4*>(&rt.rk[(uint32_t)mix64(key) & t.rk_mask]);
 }
 
 __device__ __forceinline__ void corrupt(const TableDev& t) {
@@ -159,30 +166,38 @@ __device__ __forceinline__ void corrupt(const TableDev& t) {
 // the chain's last op exchanges on the (key, batch) entry.  A mutation whose
 // predecessor is known (an earlier op of its wavefront chain, or the chain
 // exchanged before) writes its result now; the first mutation of a (key,
-// batch) chain is left to pass B (run_pred = RUN_CF | entry), and its chain's
-// last op registers the batch in the key's bitmap and lists the entry for
-// clearing (clr, one list per block, counted by ccnt in LDS).
+// batch) chain is left to pass B (run_pred = RUN_CF | entry).  The op that
+// claims a (key, batch) entry registers the batch in the key's bitmap and
+// lists the entry for clearing (clr, one list per block, counted by ccnt in
+// LDS), in parallel with its exchange.  The next op's record is loaded while
+// the current one's atomics are in flight.
 template <class L, int G>
-__device__ __forceinline__ void run_pass_a(const TableDev& t, const OpsDev& o, const OutDev& out,
-                                           uint64_t lo, uint64_t m, const RunOffs& ro,
-                                           uint64_t gid, uint64_t ng, uint32_t* clr,
-                                           uint32_t* ccnt) {
+__device__ __forceinline__ void run_pass_a(const TableDev& t, const RunTabs& rt, const OpsDev& o,
+                                           const OutDev& out, uint64_t lo, uint64_t m,
+                                           const RunOffs& ro, uint64_t gid, uint64_t ng,
+                                           uint32_t* clr, uint32_t* ccnt) {
     using K = typename L::K;
     using V = typename L::V;
     const int gl = Grp<G>::lane();
     const int lane = (int)__lane_id();
+    uint8_t op_n = OP_GET;
+    K key_n = K(0);
+    if (gid < m) {
+        op_n = ld_opcode(o, lo + gid);
+        key_n = ld_stride<K>(o.keys, o.kstride, lo + gid);
+    }
     // The groups of a wavefront have consecutive ids, so the trip count is
     // wave-uniform; the ballot keeps it so.
     for (uint64_t li = gid; __ballot(li < m) != 0; li += ng) {
         const bool act = li < m;
-        const uint8_t op = act ? ld_opcode(o, lo + li) : (uint8_t)OP_GET;
-        const bool lead = act && op != OP_GET && gl == 0;
-        K key = K(0);
-        int r = 0;
-        if (lead) {
-            key = ld_stride<K>(o.keys, o.kstride, lo + li);
-            r = run_batch_of(ro, li);
+        const uint8_t op = op_n;
+        const K key = key_n;
+        if (li + ng < m) {
+            op_n = ld_opcode(o, lo + li + ng);
+            key_n = ld_stride<K>(o.keys, o.kstride, lo + li + ng);
         }
+        const bool lead = act && op != OP_GET && gl == 0;
+        const int r = lead ? run_batch_of(ro, li) : 0;
         const uint64_t kk = (uint64_t)key;
         uint64_t todo = __ballot(lead);
         if (todo == 0) continue;  // wave-uniform
@@ -203,16 +218,27 @@ __device__ __forceinline__ void run_pass_a(const TableDev& t, const OpsDev& o, c
         }
         const bool rep = lead && lane == my_last;
         uint32_t c = NOENT, fpred = 0;
+        bool won = false;
         if (rep) {
             uint64_t base;
             uint32_t cmask;
             run_region(ro, r, base, cmask);
-            c = rc_claim(t, kk, base, cmask, r);
-            if (c != NOENT)
-                fpred = __hip_atomic_exchange(&t.rc[c].head, (uint32_t)li + 1u, __ATOMIC_RELAXED,
+            c = rc_claim(t, rt, kk, base, cmask, r, won);
+            if (c != NOENT) {
+                fpred = __hip_atomic_exchange(&rt.rc[c].head, (uint32_t)li + 1u, __ATOMIC_RELAXED,
                                               __HIP_MEMORY_SCOPE_AGENT);
-            else
+                if (kk == GT_EMPTY) won = fpred == 0;  // its dedicated entry's first exchange
+                if (won) {
+                    const uint32_t gk = rk_claim(t, rt, kk);
+                    if (gk != NOENT)
+                        __hip_atomic_fetch_or(&rt.rk[gk].mask, 1ull << r, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+                    else
+                        corrupt(t);
+                }
+            } else {
                 corrupt(t);
+            }
         }
         const uint32_t fp = __shfl(fpred, my_last, 64);
         const uint32_t cc = __shfl(c, my_last, 64);
@@ -226,16 +252,7 @@ __device__ __forceinline__ void run_pass_a(const TableDev& t, const OpsDev& o, c
                 st_coh(&t.run_pred[li], RUN_CF | cc);
             }
         }
-        const bool first_chain = rep && c != NOENT && fpred == 0;
-        if (first_chain) {
-            const uint32_t gk = rk_claim(t, kk);
-            if (gk != NOENT)
-                __hip_atomic_fetch_or(&t.rk[gk].mask, 1ull << r, __ATOMIC_RELAXED,
-                                      __HIP_MEMORY_SCOPE_AGENT);
-            else
-                corrupt(t);
-        }
-        const uint64_t fm = __ballot(first_chain);
+        const uint64_t fm = __ballot(won);
         if (fm) {
             const int l0 = __builtin_ctzll(fm);
             uint32_t wb = 0;
@@ -243,7 +260,7 @@ __device__ __forceinline__ void run_pass_a(const TableDev& t, const OpsDev& o, c
                 wb = __hip_atomic_fetch_add(ccnt, (uint32_t)__builtin_popcountll(fm),
                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             wb = __builtin_amdgcn_readlane(wb, l0);
-            if (first_chain)
+            if (won)
                 st_coh(&clr[wb + (uint32_t)__builtin_popcountll(fm & ((1ull << lane) - 1ull))], c);
         }
     }
@@ -254,98 +271,121 @@ template <class L>
 __device__ __forceinline__ void op_effect(const OpsDev& o, uint64_t i, bool& found,
                                           typename L::V& cur) {
     using V = typename L::V;
-    if (ld_opcode(o, i) == OP_PUT) {
-        found = true;
-        cur = L::IS_MAP ? ld_stride<V>(o.vals, o.vstride, i) : V(0);
-    } else {
-        found = false;
-        cur = V(0);
+    const uint8_t pop = ld_opcode(o, i);
+    const V pv = (L::IS_MAP && o.vals) ? ld_stride<V>(o.vals, o.vstride, i) : V(0);
+    found = pop == OP_PUT;
+    cur = found ? pv : V(0);
+}
+
+// One op of pass B (see run_pass_b), its record (opcode, key, value and
+// run_pred) already loaded.
+template <class L, int G>
+__device__ __forceinline__ void run_b_op(const TableDev& t, const RunTabs& rt, const OpsDev& o,
+                                         const OutDev& out, uint64_t lo, uint64_t m,
+                                         const RunOffs& ro, uint64_t li,
+                                         uint8_t op, typename L::K key, uint32_t pred, RepEnt* rb,
+                                         RepEnt* lrep, uint32_t lcap, uint32_t* rcnt) {
+    using K = typename L::K;
+    using V = typename L::V;
+    constexpr int S = PW<L, G>::N;
+    const int gl = Grp<G>::lane();
+    if (op != OP_GET && !(pred & RUN_CF)) return;  // group-uniform: written in pass A
+    const uint64_t kk = (uint64_t)key;
+    const uint64_t b = bucket_of(t, kk);
+    K blk[S];
+    probe_issue<L, G, false>(t, b, blk);
+    uint4 w0 = make_uint4(0, 0, 0, 0);
+    if (gl == 0) w0 = rk_first(t, rt, kk);
+    const int r = run_batch_of(ro, li);
+    uint64_t mask = 0;
+    uint32_t gk = NOENT;
+    if (gl == 0) gk = rk_lookup(t, rt, kk, w0, mask);
+    mask = grp_bcast(mask, 0, G);
+    gk = grp_bcast(gk, 0, G);
+    const uint64_t earlier = mask & ((1ull << r) - 1ull);
+    const bool last = op != OP_GET && (r == RUN_MAX_BATCHES - 1 || (mask >> (r + 1)) == 0);
+    V cur = V(0);
+    uint64_t s = b;
+    int slot = -1;
+    if (earlier == 0 || last) {
+        uint32_t valid;
+        walk_on<L, G, false>(t, probe_compute<L, G, false>(t, b, key, blk), s, slot, valid, key);
+        if constexpr (L::IS_MAP) {
+            if (slot >= 0 && earlier == 0 && gl == 0) cur = *val_slot<L>(t, s, slot);
+        }
     }
+    if (gl != 0) return;
+    const bool in_table = slot >= 0;
+    bool found = in_table;
+    uint32_t p = 0;  // predecessor: the tail of the nearest earlier batch's chain
+    if (earlier) {
+        const int r2 = 63 - __builtin_clzll(earlier);
+        uint64_t base;
+        uint32_t cmask;
+        run_region(ro, r2, base, cmask);
+        p = rc_tail(t, rt, kk, base, cmask, r2);
+        if (p == 0 || p > m) {
+            corrupt(t);
+            p = 0;
+        }
+    }
+    if (op == OP_GET) {
+        if (p) op_effect<L>(o, lo + p - 1u, found, cur);
+        write_result<L>(out, lo + li, found, cur);
+        return;
+    }
+    mut_result<L>(t, o, out, lo, m, (uint32_t)li, op, p, found, cur);
+    if (!last) return;
+    const uint32_t c = pred & ~RUN_CF;
+    const uint32_t fin = c < t.rc_cap + RUN_MAX_BATCHES ? rt.rc[c].head : 0u;
+    if (gk == NOENT || fin == 0 || fin > m) {
+        corrupt(t);
+        return;
+    }
+    RepEnt e;
+    e.li = fin - 1u;
+    e.g = gk;
+    e.loc = in_table ? ((s << 5) | (uint64_t)slot) : NOLOC;
+    const uint32_t k = __hip_atomic_fetch_add(rcnt, 1u, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (k < lcap) lrep[k] = e;
+    else rb[k] = e;
 }
 
 // Pass B over the run's ops: every get and every first mutation of a (key,
 // batch) chain.  The table probe and the key table's first slot are issued
 // together (both depend on the key alone); the probe finishes only when the
 // key has no earlier mutation in the run or this is the key's last chain (it
-// then gives the key's location for phase W).
+// then gives the key's location for phase W).  A record's opcode, key, value
+// and run_pred are loaded together, one op ahead.
 template <class L, int G>
-__device__ __forceinline__ void run_pass_b(const TableDev& t, const OpsDev& o, const OutDev& out,
-                                           uint64_t lo, uint64_t m, const RunOffs& ro,
+__device__ __forceinline__ void run_pass_b(const TableDev& t, const RunTabs& rt, const OpsDev& o,
+                                           const OutDev& out, uint64_t lo, uint64_t m,
+                                           const RunOffs& ro,
                                            uint64_t gid, uint64_t ng, RepEnt* rb, RepEnt* lrep,
                                            uint32_t lcap, uint32_t* rcnt) {
     using K = typename L::K;
-    using V = typename L::V;
-    constexpr int S = PW<L, G>::N;
-    const int gl = Grp<G>::lane();
-    for (uint64_t li = gid; li < m; li += ng) {
-        uint8_t op;
-        K key;
-        V val;
-        load_op<L>(o, lo + li, op, key, val);
-        const uint32_t pred = op != OP_GET ? t.run_pred[li] : RUN_CF;
-        if (!(pred & RUN_CF)) continue;  // group-uniform: written in pass A
-        const uint64_t kk = (uint64_t)key;
-        const uint64_t b = bucket_of(t, kk);
-        K blk[S];
-        probe_issue<L, G, false>(t, b, blk);
-        uint4 w0 = make_uint4(0, 0, 0, 0);
-        if (gl == 0) w0 = rk_first(t, kk);
-        const int r = run_batch_of(ro, li);
-        uint64_t mask = 0;
-        uint32_t gk = NOENT;
-        if (gl == 0) gk = rk_lookup(t, kk, w0, mask);
-        mask = grp_bcast(mask, 0, G);
-        gk = grp_bcast(gk, 0, G);
-        const uint64_t earlier = mask & ((1ull << r) - 1ull);
-        const bool last = op != OP_GET && (r == RUN_MAX_BATCHES - 1 || (mask >> (r + 1)) == 0);
-        bool found = false;
-        V cur = V(0);
-        uint64_t s = b;
-        int slot = -1;
-        if (earlier == 0 || last) {
-            uint32_t valid;
-            walk_on<L, G, false>(t, probe_compute<L, G, false>(t, b, key, blk), s, slot, valid, key);
-            if constexpr (L::IS_MAP) {
-                if (slot >= 0 && earlier == 0 && gl == 0) cur = *val_slot<L>(t, s, slot);
-            }
-        }
-        if (gl != 0) continue;
-        const bool in_table = slot >= 0;
-        found = in_table;
-        uint32_t p = 0;  // predecessor: the tail of the nearest earlier batch's chain
-        if (earlier) {
-            const int r2 = 63 - __builtin_clzll(earlier);
-            uint64_t base;
-            uint32_t cmask;
-            run_region(ro, r2, base, cmask);
-            p = rc_tail(t, kk, base, cmask, r2);
-            if (p == 0 || p > m) {
-                corrupt(t);
-                p = 0;
-            }
-        }
-        if (op == OP_GET) {
-            if (p) op_effect<L>(o, lo + p - 1u, found, cur);
-            write_result<L>(out, lo + li, found, cur);
-            continue;
-        }
-        mut_result<L>(t, o, out, lo, m, (uint32_t)li, op, p, found, cur);
-        if (last) {
-            const uint32_t c = pred & ~RUN_CF;
-            const uint32_t fin = c < t.rc_cap + RUN_MAX_BATCHES ? t.rc[c].head : 0u;
-            if (gk == NOENT || fin == 0 || fin > m) {
-                corrupt(t);
-                continue;
-            }
-            RepEnt e;
-            e.li = fin - 1u;
-            e.g = gk;
-            e.loc = in_table ? ((s << 5) | (uint64_t)slot) : NOLOC;
-            const uint32_t k = __hip_atomic_fetch_add(rcnt, 1u, __ATOMIC_RELAXED,
-                                                      __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (k < lcap) lrep[k] = e;
-            else rb[k] = e;
-        }
+    if (gid >= m) return;  // group-uniform
+    auto load = [&](uint64_t li, uint8_t& op, K& key, uint32_t& pred) {
+        op = ld_opcode(o, lo + li);
+        key = ld_stride<K>(o.keys, o.kstride, lo + li);
+        pred = t.run_pred[li];  // meaningful for mutations only
+    };
+    uint8_t op;
+    K key;
+    uint32_t pred;
+    load(gid, op, key, pred);
+    for (uint64_t li = gid;; li += ng) {
+        const uint64_t ln = li + ng;
+        uint8_t op_n = OP_GET;
+        K key_n = K(0);
+        uint32_t pred_n = 0;
+        if (ln < m) load(ln, op_n, key_n, pred_n);
+        run_b_op<L, G>(t, rt, o, out, lo, m, ro, li, op, key, pred, rb, lrep, lcap, rcnt);
+        if (ln >= m) break;
+        op = op_n;
+        key = key_n;
+        pred = pred_n;
     }
 }
 
@@ -353,7 +393,8 @@ __device__ __forceinline__ void run_pass_b(const TableDev& t, const OpsDev& o, c
 // key's final state and empties its key-table entry; each listed chain entry
 // is emptied.
 template <class L, int G, bool RC>
-__device__ __forceinline__ void run_phase_w(const TableDev& t, const OpsDev& o, uint64_t lo,
+__device__ __forceinline__ void run_phase_w(const TableDev& t, const RunTabs& rt, const OpsDev& o,
+                                            uint64_t lo,
                                             const RepEnt* rb, uint32_t cnt, const RepEnt* lrep,
                                             uint32_t lcap, const uint32_t* clr, uint32_t ccnt) {
     using K = typename L::K;
@@ -367,16 +408,16 @@ __device__ __forceinline__ void run_phase_w(const TableDev& t, const OpsDev& o, 
         const K key = ld_stride<K>(o.keys, o.kstride, i);
         const V val = (L::IS_MAP && o.vals) ? ld_stride<V>(o.vals, o.vstride, i) : V(0);
         if (gl == 0) {
-            st_coh(&t.rk[e.g].key, GT_EMPTY);
-            st_coh(&t.rk[e.g].mask, (uint64_t)0);
+            st_coh(&rt.rk[e.g].key, GT_EMPTY);
+            st_coh(&rt.rk[e.g].mask, (uint64_t)0);
         }
         w_apply<L, G, RC>(t, op, key, val, e.loc);
     }
     for (uint32_t k = threadIdx.x; k < ccnt; k += blockDim.x) {
         const uint32_t c = ld_coh(&clr[k]);
         if (c < t.rc_cap + RUN_MAX_BATCHES) {
-            st_coh(&t.rc[c].key, GT_EMPTY);
-            st_coh(&t.rc[c].head, 0u);
+            st_coh(&rt.rc[c].key, GT_EMPTY);
+            st_coh(&rt.rc[c].head, 0u);
         }
     }
 }

@@ -86,6 +86,15 @@ struct RunChain {
     uint32_t pad;
 };
 
+// One set of run tables (the batched kernel alternates two, so that one
+// run's write-back overlaps the next run's first pass).
+struct RunTabs {
+    RunKey* rk;     // key table: rk_mask + 2 entries (the last: the all-ones key)
+    RunChain* rc;   // chain tables: rc_cap entries, then one per batch of a run
+                    // for the all-ones key
+    uint32_t* clr;  // chain entries to clear, one list per block (as rep)
+};
+
 struct TableDev {
     char* slabs;           // (nbuckets + pool) * SLAB_BYTES, 256-B aligned, then
                            // (u32 sets) one u32 next link per slab
@@ -103,12 +112,9 @@ struct TableDev {
     uint32_t hash_mix;     // bucket by mix64(key) instead of the key itself
     uint64_t* free_head;   // reclaim: free pool slabs, (tag << 32 | slab), 0 = empty
     uint64_t* free_count;  // reclaim: slabs on the free list
-    RunKey* rk;            // run key table: rk_mask + 2 entries (the last: the all-ones key)
-    RunChain* rc;          // run chain tables: rc_cap entries, then one per batch of a
-                           // run for the all-ones key
+    RunTabs run[2];        // run tables, used by runs of alternating parity
     uint32_t* run_pred;    // per op of a run: chain predecessor (op index + 1) or
                            // RUN_CF | chain entry for the first mutation of a chain
-    uint32_t* run_clr;     // chain entries to clear, one list per block (as rep)
     uint64_t rc_cap;       // 4 * (ops a run may hold)
     uint32_t rk_mask;      // key-table capacity - 1 (at least 2 * ops a run may hold)
 };
