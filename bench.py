@@ -17,8 +17,10 @@ The JSON line also carries the plain rate, the roofline of the fused kernel
 host memory (the table behind PCIe copies vs the KVCG hot/cold split), the batched
 form (4096 uneven batches, one graph-captured launch vs one launch per batch),
 the served form (device-resident request loop: capacity, then p50/p99 at 70%
-of it) and the host-core baseline (the oracle's sequential replay of a bounded
-sample).
+of it), the reference's own harness workload (1024-key get and put calls,
+launch-inclusive latency) and the host-core baseline (the oracle's sequential
+replay of a bounded sample).  The plain rates use the plain engine's own tuned
+lanes per op.
 Launch shapes come from lslab_amd/tuning.json (python -m lslab_amd.autotune).
 """
 import argparse
@@ -214,9 +216,10 @@ def time_fused(m, ops, keys, vals, offs, nb, torch):
     return wall, e0.elapsed_time(e1) / 1e3
 
 
-def batched(lslab_amd, wl, torch, m, tun, nbatches=4096):
+def batched(lslab_amd, wl, torch, m, tun, plain_lanes, nbatches=4096):
     """Config 4 on the prefilled table: one graph-captured persistent launch
-    vs one plain launch per batch."""
+    vs one plain launch per batch (at the plain engine's own tuned lanes per
+    op)."""
     from lslab_amd import _lib
     lib = _lib.load()
     offs_h, ops, keys, vals = gen_batched(wl, nbatches)
@@ -225,7 +228,7 @@ def batched(lslab_amd, wl, torch, m, tun, nbatches=4096):
     offs = torch.from_numpy(offs_h).cuda()
     out = torch.empty((n, 2), dtype=torch.int64, device="cuda")
     sp = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
-    m.set_tuning(lslab_amd.tuning(lanes_per_op=tun["lanes_per_op"], engine=lslab_amd.ENGINE_PLAIN,
+    m.set_tuning(lslab_amd.tuning(lanes_per_op=plain_lanes, engine=lslab_amd.ENGINE_PLAIN,
                                   max_subbatch=BATCH))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -263,7 +266,7 @@ def batched(lslab_amd, wl, torch, m, tun, nbatches=4096):
             "fused_graph_mops": round(n / fused_s / 1e6, 2),
             "fused_vs_plain": round(plain_s / fused_s, 3),
             "lanes_per_op": tun["lanes_per_op"], "max_blocks": tun["max_blocks"],
-            "combine": tun["combine"]}
+            "combine": tun["combine"], "plain_lanes_per_op": plain_lanes}
 
 
 def host_input(lslab_amd, torch, m, tun, ops, keys, vals, nb, hot_keys=1 << 16):
@@ -451,36 +454,104 @@ def other_configs(lslab_amd, wl, torch):
         res[f"u32_map_put_{name}_mops"] = round(n / put_s / 1e6, 1)
         res[f"u32_map_get_{name}_mops"] = round(n / get_s / 1e6, 1)
     del keys, vals, absent, queries
-    mut, look = lslab_amd.tuning(lanes_per_op=8), lslab_amd.tuning(lanes_per_op=2)
-    s = lslab_amd.Set(8, 24, 1 << 16, mut)
-    total, chunk = 1 << 28, 1 << 24
-    ins_s = 0.0
-    for lo in range(0, total, chunk):
-        k = wl.keys(chunk, 8, first=lo, seed=0)
-        ins_s += _timed(torch, lambda: s.insert(k))
-        del k
-    q = 1 << 26
-    qk = torch.cat([wl.keys(q * 4 // 5, 8, first=0, seed=0),
-                    wl.keys(q - q * 4 // 5, 8, first=total, seed=0)])
-    s.set_tuning(look)
-    got = {}
-    con_s = _timed(torch, lambda: got.setdefault("r", s.contains(qk)))
-    if int(got["r"].sum()) != q * 4 // 5:
-        raise RuntimeError("set at scale: wrong contains results")
-    del qk, got
-    s.set_tuning(mut)
-    rm_s, removed = 0.0, 0
-    for lo in range(0, total, chunk):
-        k = wl.keys(chunk, 8, first=lo, seed=0)[::2].contiguous()  # half of the keys
-        rm_s += _timed(torch, lambda: s.remove(k))
-        removed += k.numel()
-        del k
-    if s.status() or s.stats()["present"] != total - removed:
-        raise RuntimeError("set at scale: wrong contents")
-    s.close()
-    res.update({"set_insert_mops": round(total / ins_s / 1e6, 1),
-                "set_contains_mops": round(q / con_s / 1e6, 1),
-                "set_remove_mops": round(removed / rm_s / 1e6, 1)})
+    look = lslab_amd.tuning(lanes_per_op=2)
+    for engine, name, lanes in ((lslab_amd.ENGINE_PLAIN, "plain", 8),
+                                (lslab_amd.ENGINE_TWO_PHASE, "two_phase", 4)):
+        mut = lslab_amd.tuning(lanes_per_op=lanes, engine=engine, max_subbatch=1 << 24)
+        s = lslab_amd.Set(8, 24, 1 << 16, mut)
+        total, chunk = 1 << 28, 1 << 24
+        ins_s = 0.0
+        for lo in range(0, total, chunk):
+            k = wl.keys(chunk, 8, first=lo, seed=0)
+            ins_s += _timed(torch, lambda: s.insert(k))
+            del k
+        q = 1 << 26
+        qk = torch.cat([wl.keys(q * 4 // 5, 8, first=0, seed=0),
+                        wl.keys(q - q * 4 // 5, 8, first=total, seed=0)])
+        s.set_tuning(look)
+        got = {}
+        con_s = _timed(torch, lambda: got.setdefault("r", s.contains(qk)))
+        if int(got["r"].sum()) != q * 4 // 5:
+            raise RuntimeError("set at scale: wrong contains results")
+        del qk, got
+        s.set_tuning(mut)
+        rm_s, removed = 0.0, 0
+        for lo in range(0, total, chunk):
+            k = wl.keys(chunk, 8, first=lo, seed=0)[::2].contiguous()  # half of the keys
+            rm_s += _timed(torch, lambda: s.remove(k))
+            removed += k.numel()
+            del k
+        st = s.stats()
+        if s.status() or st["present"] != total - removed:
+            raise RuntimeError("set at scale: wrong contents")
+        s.close()
+        res.update({f"set_insert_{name}_mops": round(total / ins_s / 1e6, 1),
+                    f"set_contains_mops": round(q / con_s / 1e6, 1),
+                    f"set_remove_{name}_mops": round(removed / rm_s / 1e6, 1)})
+    return res
+
+
+def reference_harness(lslab_amd, torch, reps=100):
+    """The reference's own benchmark (benchmark/benchmark.cu:18-177):
+    map<u64,u32> with 2^20 buckets, keys 1..2^19 put with value 0, then
+    batches of 1024 keys uniform over [0, 2^21] (a seeded generator; the
+    reference draws rand()), first gets, then puts of value 0, one call per
+    batch timed by wall clock around the launch and the stream
+    synchronisation, as the reference times it; the kernel time from HIP
+    events on the same stream is reported beside it.  reps batches per
+    series (the reference prints 10); puts with both engines."""
+    import numpy as np
+    from lslab_amd import _lib
+    lib = _lib.load()
+    pop, rng_hi, batch = 1 << 19, 1 << 21, 1024
+    res = {"workload": "map<u64,u32> 2^20 buckets, keys 1..2^19 prefilled; batches of 1024 "
+                       "keys uniform over [0, 2^21]; one call per batch",
+           "reps": reps}
+    s = torch.cuda.current_stream()
+    sp = ctypes.c_void_p(s.cuda_stream)
+    g = np.random.default_rng(0xBE7C)
+    kb = [torch.from_numpy(g.integers(0, rng_hi + 1, size=batch, dtype=np.int64)).cuda()
+          for _ in range(reps + 3)]
+    zero = torch.zeros(batch, dtype=torch.int32, device="cuda")
+    pairs = torch.empty((batch, 2), dtype=torch.int32, device="cuda")
+    old = torch.empty(batch, dtype=torch.int32, device="cuda")
+
+    def series(call):
+        wall, kern = [], []
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        for i in range(reps + 3):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            e0.record(s)
+            _lib.check(call(kb[i]), "reference harness")
+            e1.record(s)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            if i >= 3:  # the first calls warm the launch path
+                wall.append(dt)
+                kern.append(e0.elapsed_time(e1) / 1e3)
+        w = np.array(wall)
+        return {"latency_us_p50": round(float(np.percentile(w, 50)) * 1e6, 2),
+                "latency_us_p99": round(float(np.percentile(w, 99)) * 1e6, 2),
+                "mops_p50": round(batch / float(np.percentile(w, 50)) / 1e6, 2),
+                "kernel_us_p50": round(float(np.percentile(kern, 50)) * 1e6, 2)}
+
+    for engine, name in ((lslab_amd.ENGINE_PLAIN, "plain"), (lslab_amd.ENGINE_TWO_PHASE, "two_phase")):
+        m = lslab_amd.Map(8, 4, 20, 10000, lslab_amd.tuning(lanes_per_op=8, engine=engine,
+                                                           max_subbatch=1 << 19))
+        m.put(torch.arange(1, pop + 1, dtype=torch.int64, device="cuda"),
+              torch.zeros(pop, dtype=torch.int32, device="cuda"))
+        if name == "plain":
+            m.set_tuning(lslab_amd.tuning(lanes_per_op=2, engine=engine, max_subbatch=1 << 19))
+            res["get"] = series(lambda k: lib.lslab_map_get(
+                m.handle, ctypes.c_void_p(k.data_ptr()), ctypes.c_void_p(pairs.data_ptr()), batch, sp))
+            m.set_tuning(lslab_amd.tuning(lanes_per_op=8, engine=engine, max_subbatch=1 << 19))
+        res[f"put_{name}"] = series(lambda k: lib.lslab_map_put(
+            m.handle, ctypes.c_void_p(k.data_ptr()), ctypes.c_void_p(zero.data_ptr()),
+            ctypes.c_void_p(old.data_ptr()), batch, sp))
+        if m.status():
+            raise RuntimeError("reference harness status")
+        m.close()
     return res
 
 
@@ -571,6 +642,7 @@ def main():
     ap.add_argument("--no-host-input", action="store_true")
     ap.add_argument("--no-wide", action="store_true")
     ap.add_argument("--no-configs", action="store_true")
+    ap.add_argument("--no-reference-harness", action="store_true")
     ap.add_argument("--served-requests", type=int, default=20000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 1 GiB device copy first, as the HBM byte-counter reference")
@@ -587,7 +659,8 @@ def main():
 
     torch.cuda.set_device(0)
     K, W = args.steps, args.warmup
-    tuned = {k: autotune.load(k) for k in ("headline", "batched", "served")}
+    tuned = {k: autotune.load(k) for k in ("headline", "batched", "served", "headline_plain",
+                                           "batched_plain")}
     for v in tuned.values():
         v.setdefault("combine", lslab_amd.COMBINE_BLOCK)
         if args.lanes:
@@ -611,8 +684,9 @@ def main():
         torch.cuda.synchronize()
         del rows, idx, picked
     th = tuned["headline"]
+    plain_lanes = tuned["headline_plain"]["lanes_per_op"]
     m = lslab_amd.Map(8, 8, N_LOG2, 1 << 16,
-                      lslab_amd.tuning(lanes_per_op=th["lanes_per_op"],
+                      lslab_amd.tuning(lanes_per_op=plain_lanes,
                                        engine=lslab_amd.ENGINE_PLAIN, max_subbatch=BATCH))
     t0 = time.perf_counter()
     prefill(m, wl)
@@ -644,6 +718,7 @@ def main():
     log(f"fused: {wall_f * 1e3:.2f} ms wall, {kern_f * 1e3:.2f} ms kernel -> {fused_mops:.1f} Mops/s")
 
     achieved = B_OP * K * BATCH / kern_f / 1e9
+    tpo = measured_traffic_per_op()
     per_sorted = sorted(per_p)
     res = {
         "metric": METRIC,
@@ -670,6 +745,9 @@ def main():
         },
         "plain_mops": round(plain_mops, 2),
         "fused_vs_plain": round(fused_mops / plain_mops, 3),
+        "plain_best_lanes": plain_lanes,
+        "plain_lanes_sweep": {str(e["lanes_per_op"]): e["mops"]
+                              for e in tuned["headline_plain"].get("all", [])},
         "plain_batch_ms_p50": round(per_sorted[len(per_sorted) // 2], 4),
         "plain_batch_ms_p99": round(per_sorted[min(len(per_sorted) - 1,
                                                    int(0.99 * len(per_sorted)))], 4),
@@ -679,8 +757,12 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": (round(measured_traffic_per_op() * K * BATCH)
-                        if measured_traffic_per_op() else None),
+            "traffic": round(tpo * K * BATCH) if tpo else None,
+            # the HBM byte rate the launch actually moved (rocprof bytes per op x
+            # ops/s): the kernel is bound by dependent random accesses, not bytes
+            "measured_gbs": round(tpo * K * BATCH / kern_f / 1e9, 1) if tpo else None,
+            "measured_frac": round(tpo * K * BATCH / kern_f / 1e9 / HBM_PEAK_GBS, 4) if tpo else None,
+            "achieved_is": "algorithmic bytes (reference slab layout) / kernel time",
             "traffic_source": "rocprofv3 FETCH_SIZE (x1.063, random-line calibration) + WRITE_SIZE per op "
                               "of k_batches, profiles/k_batches_map_u64_zipf099_headline.md, "
                               "times the ops of this launch",
@@ -703,9 +785,14 @@ def main():
         log(res["wide"])
     if not args.no_batched:
         log("batched form (4096 uneven batches) ...")
-        res["batched"] = batched(lslab_amd, wl, torch, m, tuned["batched"])
+        res["batched"] = batched(lslab_amd, wl, torch, m, tuned["batched"],
+                                 tuned["batched_plain"]["lanes_per_op"])
         log(res["batched"])
     m.close()
+    if not args.no_reference_harness:
+        log("reference harness workload (1024-key batches) ...")
+        res["reference_harness"] = reference_harness(lslab_amd, torch)
+        log(res["reference_harness"])
     if not args.no_served:
         log("served form ...")
         sv = served(lslab_amd, wl, torch, tuned["served"], args.served_requests)

@@ -95,6 +95,10 @@ typedef struct {
     int hash;              /* LSLAB_HASH_* at create (0 = IDENTITY); set_tuning keeps it
                               (0 there means "unchanged", another value is an error) */
     int reclaim;           /* LSLAB_RECLAIM_* at create (0 = OFF); set_tuning keeps it */
+    int spin_ms;           /* bound on every device spin -- a bucket lock, a grid barrier --
+                              in ms, 10..3600000 (0 = 10000): a spin that makes no progress
+                              for that long ends its kernel with LSLAB_ERR_TIMEOUT.  The
+                              served loop's wait for an arrival is not bounded by it. */
 } lslab_tuning;
 
 typedef struct {

@@ -65,6 +65,7 @@ class Tuning(ctypes.Structure):
         ("combine", ctypes.c_int),
         ("hash", ctypes.c_int),
         ("reclaim", ctypes.c_int),
+        ("spin_ms", ctypes.c_int),
     ]
 
 

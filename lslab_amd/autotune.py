@@ -12,7 +12,10 @@ For each timed workload it sweeps, on a short sample of that workload:
   - combine      : combining scope of same-key mutations before the global
                    exchange (1 = within a wavefront, 2 = across the block);
   - tick ops     : (served form) ops admitted per tick.
-and keeps the setting with the highest Mops/s.  Results go to
+and keeps the setting with the highest Mops/s.  It also sweeps lanes_per_op of
+the plain per-batch engine (bucket locks, 1..32 lanes, 32 being the
+reference's warp-per-op shape) on the headline and batched streams, the
+baselines the fused forms are compared with.  Results go to
 lslab_amd/tuning.json under the workload's name; bench.py reads them.
 
 python -m lslab_amd.autotune
@@ -29,9 +32,13 @@ TUNING_JSON = os.path.join(HERE, "tuning.json")
 
 DEFAULTS = {
     "headline": {"lanes_per_op": 1, "max_blocks": 0, "combine": 1},
-    "batched": {"lanes_per_op": 8, "max_blocks": 128, "combine": 2},
+    "batched": {"lanes_per_op": 1, "max_blocks": 0, "combine": 1},
     "served": {"lanes_per_op": 2, "max_blocks": 128, "tick_ops": 1 << 18, "combine": 2},
+    # the plain per-batch engine (bucket locks) on the same two streams
+    "headline_plain": {"lanes_per_op": 2},
+    "batched_plain": {"lanes_per_op": 2},
 }
+PLAIN_LANES = (1, 2, 4, 8, 16, 32)  # 32: one half-wave per op, the reference's warp per op
 
 
 def load(name):
@@ -88,6 +95,35 @@ def search_batches(table, offs, ops, keys, vals, nb, n_ops, max_subbatch, lanes,
     return max(res, key=lambda r: r["mops"]), res
 
 
+def search_plain(table, offs_h, ops, keys, vals, nb, log, name):
+    """Lanes per op of the plain engine: each batch one locked launch, back to
+    back on one stream, as the benchmark times it."""
+    import torch
+    from . import _lib, table as tb
+    lib = _lib.load()
+    n = int(offs_h[nb])
+    out = torch.empty((n, 2), dtype=torch.int64, device="cuda")
+    sp = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    res = []
+    for g in PLAIN_LANES:
+        table.set_tuning(tb.tuning(lanes_per_op=g, engine=_lib.ENGINE_PLAIN, max_subbatch=1 << 20))
+
+        def run():
+            for b in range(nb):
+                lo = int(offs_h[b])
+                _lib.check(lib.lslab_map_mixed(table.handle, ctypes.c_void_p(ops[lo:].data_ptr()),
+                                               ctypes.c_void_p(keys[lo:].data_ptr()),
+                                               ctypes.c_void_p(vals[lo:].data_ptr()),
+                                               ctypes.c_void_p(out[lo:].data_ptr()),
+                                               int(offs_h[b + 1] - lo), sp), "map_mixed")
+        dt = _timed(run, reps=1)
+        if table.status():
+            raise RuntimeError("status during the plain sweep")
+        res.append({"lanes_per_op": g, "mops": round(n / dt / 1e6, 1)})
+        log(f"[autotune] {name} plain lanes={g}: {res[-1]['mops']} Mops/s")
+    return max(res, key=lambda r: r["mops"]), res
+
+
 def main():
     sys.path.insert(0, os.path.dirname(HERE))
     import numpy as np
@@ -108,12 +144,16 @@ def main():
     top, res = search_batches(m, offs, ops, keys, vals, nb, nb * bench.BATCH, bench.BATCH,
                               (1, 2, 4, 8), (0, 192, 128), log)
     out["headline"] = dict(top, all=res)
+    top, res = search_plain(m, offs.cpu().numpy(), ops, keys, vals, 3, log, "headline")
+    out["headline_plain"] = dict(top, all=res)
     # batched: the first 1024 of the uneven batches
     offs_h, ops, keys, vals = bench.gen_batched(wl, 1024)
     offs = torch.from_numpy(offs_h).cuda()
     top, res = search_batches(m, offs, ops, keys, vals, 1024, int(offs_h[-1]), bench.BATCH,
-                              (2, 4, 8), (0, 128, 64, 32), log)
+                              (1, 2, 4, 8), (0, 128, 64), log)
     out["batched"] = dict(top, all=res)
+    top, res = search_plain(m, offs_h, ops, keys, vals, 256, log, "batched")
+    out["batched_plain"] = dict(top, all=res)
     m.close()
     # served: loop capacity (every request arrives at t=0) on a short stream
     res = []

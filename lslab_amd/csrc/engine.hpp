@@ -402,7 +402,7 @@ __device__ __forceinline__ void run_op(const TableDev& t, uint8_t op, typename L
                 } else {
                     const uint64_t now = now_ticks();
                     if (t0 == 0) t0 = now;
-                    if (now - t0 > SPIN_LIMIT_TICKS) {
+                    if (now - t0 > t.spin_ticks) {
                         if (gl == 0) __hip_atomic_fetch_or(t.status, ST_TIMEOUT, __ATOMIC_RELAXED,
                                                            __HIP_MEMORY_SCOPE_AGENT);
                         rfound = false;
@@ -896,7 +896,7 @@ __device__ __forceinline__ void w_insert(const TableDev& t, typename L::K key, t
                     busy = 1;
                     const uint64_t now = now_ticks();
                     if (t0 == 0) t0 = now;
-                    if (now - t0 > SPIN_LIMIT_TICKS) {
+                    if (now - t0 > t.spin_ticks) {
                         __hip_atomic_fetch_or(t.status, ST_TIMEOUT, __ATOMIC_RELAXED,
                                               __HIP_MEMORY_SCOPE_AGENT);
                         done = 1;

@@ -37,7 +37,8 @@ struct BarState {
     BarWord top;
     BarWord group[BAR_GROUPS];
     uint32_t abort;
-    uint32_t pad[31];
+    uint32_t beat;  // heartbeat of a live wait elsewhere (the served loop's idle scheduler)
+    uint32_t pad[30];
 };
 
 // Plain per-batch kernel: one G-lane group per op, grid-stride.  RC: the
@@ -94,10 +95,12 @@ __global__ __launch_bounds__(256) void k_phase_w(TableDev t, OpsDev o, uint64_t 
 // shared state only through agent-scope loads, or data its own CU wrote,
 // passes acquire = false.  No cache-wide write-back is issued.  Spins are
 // bounded; on timeout or abort the barrier returns false and the caller
-// leaves the kernel.  reset, reset2 (optional): LDS words that thread 0
-// zeroes while the whole block is inside the barrier.
-__device__ __forceinline__ bool grid_sync(BarState* bar, uint32_t nblocks, bool acquire,
-                                          uint32_t* reset = nullptr, uint32_t* reset2 = nullptr) {
+// leaves the kernel (limit: ticks without progress, see the heartbeat
+// below).  reset, reset2 (optional): LDS words that thread 0 zeroes while the
+// whole block is inside the barrier.
+__device__ __forceinline__ bool grid_sync(BarState* bar, uint32_t nblocks, uint64_t limit,
+                                          bool acquire, uint32_t* reset = nullptr,
+                                          uint32_t* reset2 = nullptr) {
     wait_vm();
     __syncthreads();
     if (reset && threadIdx.x == 0) *reset = 0;
@@ -121,9 +124,19 @@ __device__ __forceinline__ bool grid_sync(BarState* bar, uint32_t nblocks, bool 
                                            __HIP_MEMORY_SCOPE_AGENT);
             }
         }
-        const uint64_t t0 = now_ticks();
+        // The bound restarts whenever the heartbeat moves: a block that waits
+        // here while the served loop's scheduler waits for an arrival is not
+        // stuck, however long the gap.
+        uint64_t t0 = now_ticks();
+        uint32_t seen = ld_agent(&bar->beat);
         while ((int32_t)(ld_agent(&gw->gen) - inst) <= 0) {
-            if (ld_agent(&bar->abort) || now_ticks() - t0 > SPIN_LIMIT_TICKS) {
+            const uint64_t now = now_ticks();
+            const uint32_t beat = ld_agent(&bar->beat);
+            if (beat != seen) {
+                seen = beat;
+                t0 = now;
+            }
+            if (ld_agent(&bar->abort) || now - t0 > limit) {
                 st_coh(&bar->abort, 1u);
                 ok = 0;
                 break;
@@ -207,16 +220,16 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_batches(TableDev t, OpsDev 
                 const RunTabs pt = par ? t.run[0] : t.run[1];
                 run_phase_w<L, G, RC>(t, pt, o, plo, rb, *rc, rep_lds(), REP_LDS,
                                       pt.clr + clr_base, cc[par ^ 1]);
-                if (!grid_sync(bar, gridDim.x, true, rc, &cc[par ^ 1])) return fail();
+                if (!grid_sync(bar, gridDim.x, t.spin_ticks, true, rc, &cc[par ^ 1])) return fail();
                 pend = false;
             }
             for (uint64_t slo = lo; slo < hi; slo += submax) {
                 const uint64_t m = min(submax, hi - slo);
                 phase_r_range<L, G, PERSIST_THREADS, BC>(t, o, out, slo, m, gid, ng, rb,
                                                          rep_lds(), REP_LDS);
-                if (!grid_sync(bar, gridDim.x, false)) return fail();
+                if (!grid_sync(bar, gridDim.x, t.spin_ticks, false)) return fail();
                 phase_w_block<L, G, RC>(t, o, slo, rb, *rc, rep_lds(), REP_LDS);
-                if (!grid_sync(bar, gridDim.x, true, rc)) return fail();
+                if (!grid_sync(bar, gridDim.x, t.spin_ticks, true, rc)) return fail();
             }
             __syncthreads();  // every wave has read the window before it is reloaded
             b = b + 1;
@@ -234,9 +247,9 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_batches(TableDev t, OpsDev 
         run_pass_a<L, G>(t, rt, o, out, lo, m, *ro, gid, ng, rt.clr + clr_base, &cc[par]);
         // pass B reads the table (after that write-back) and the run tables
         // with plain loads: drop stale lines
-        if (!grid_sync(bar, gridDim.x, true, rc, &cc[par ^ 1])) return fail();
+        if (!grid_sync(bar, gridDim.x, t.spin_ticks, true, rc, &cc[par ^ 1])) return fail();
         run_pass_b<L, G>(t, rt, o, out, lo, m, *ro, gid, ng, rb, rep_lds(), REP_LDS, rc);
-        if (!grid_sync(bar, gridDim.x, false)) return fail();
+        if (!grid_sync(bar, gridDim.x, t.spin_ticks, false)) return fail();
         pend = true;
         plo = lo;
         par ^= 1;
@@ -246,7 +259,7 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_batches(TableDev t, OpsDev 
         const RunTabs pt = par ? t.run[0] : t.run[1];
         run_phase_w<L, G, RC>(t, pt, o, plo, rb, *rc, rep_lds(), REP_LDS, pt.clr + clr_base,
                               cc[par ^ 1]);
-        if (!grid_sync(bar, gridDim.x, true, rc, &cc[par ^ 1])) return fail();
+        if (!grid_sync(bar, gridDim.x, t.spin_ticks, true, rc, &cc[par ^ 1])) return fail();
     }
 }
 
@@ -303,7 +316,8 @@ struct ServeState {
 
 constexpr uint32_t SERVE_REJECTED = 0xffffffffu;
 
-__device__ inline void serve_schedule(const TableDev& t, const ServeDev& sv, ServeState* ss) {
+__device__ inline void serve_schedule(const TableDev& t, const ServeDev& sv, ServeState* ss,
+                                      BarState* bar) {
     uint64_t r = ss->next_req;
     const uint64_t used = ld_agent(t.alloc_ctr) - ld_agent(t.free_count);
     uint64_t reserved = 0, ops = 0, n = 0, lo = r;
@@ -311,10 +325,20 @@ __device__ inline void serve_schedule(const TableDev& t, const ServeDev& sv, Ser
         const uint64_t el = now_ticks() - ss->t0;
         if (sv.arrival[r] > el) {
             if (n > 0) break;
-            // Idle: wait for the next arrival (the gap is bounded by the caller).
-            const uint64_t start = now_ticks();
-            while (now_ticks() - ss->t0 < sv.arrival[r] && now_ticks() - start < SPIN_LIMIT_TICKS / 2)
-                __builtin_amdgcn_s_sleep(2);
+            // Idle: wait for the next arrival, however far away, with a
+            // heartbeat every millisecond so that the blocks waiting in the
+            // grid barrier meanwhile do not take the loop for stuck.
+            uint64_t last = now_ticks();
+            for (;;) {
+                const uint64_t now = now_ticks();
+                if (now - ss->t0 >= sv.arrival[r]) break;
+                if (now - last >= TICKS_PER_MS) {
+                    last = now;
+                    __hip_atomic_fetch_add(&bar->beat, 1u, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                }
+                __builtin_amdgcn_s_sleep(8);
+            }
             continue;
         }
         const uint64_t sz = sv.off[r + 1] - sv.off[r];
@@ -357,8 +381,8 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_serve(TableDev t, OpsDev o,
         __syncthreads();
     if (blockIdx.x == 0 && threadIdx.x == 0) ss->t0 = now_ticks();
     for (uint64_t tick = 0;; ++tick) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) serve_schedule(t, sv, ss);
-        if (!grid_sync(bar, gridDim.x, false)) goto fail;
+        if (blockIdx.x == 0 && threadIdx.x == 0) serve_schedule(t, sv, ss, bar);
+        if (!grid_sync(bar, gridDim.x, t.spin_ticks, false)) goto fail;
         {
             if (ld_agent(&ss->done)) break;
             const uint64_t r_lo = ld_coh(&ss->r_lo), r_hi = ld_coh(&ss->r_hi);
@@ -367,7 +391,7 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_serve(TableDev t, OpsDev o,
                 const uint64_t m = min(submax, hi - slo);
                 phase_r_range<L, G, PERSIST_THREADS, BC>(t, o, out, slo, m, gid, ng, rb, rep_lds(),
                                                      REP_LDS);
-                if (!grid_sync(bar, gridDim.x, false)) goto fail;
+                if (!grid_sync(bar, gridDim.x, t.spin_ticks, false)) goto fail;
                 // Every result of the tick is written in phase R: its requests
                 // complete here.  Phase W only brings the table up to date for
                 // the next tick.
@@ -382,7 +406,7 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_serve(TableDev t, OpsDev o,
                     if (threadIdx.x == 0) ss->ticks = tick + 1;
                 }
                 phase_w_block<L, G, RC>(t, o, slo, rb, *rc, rep_lds(), REP_LDS);
-                if (!grid_sync(bar, gridDim.x, true, rc)) goto fail;
+                if (!grid_sync(bar, gridDim.x, t.spin_ticks, true, rc)) goto fail;
             }
         }
     }

@@ -136,6 +136,7 @@ static void normalise(lslab_tuning& tu) {
     if (tu.combine == 0) tu.combine = LSLAB_COMBINE_BLOCK;
     if (tu.hash == 0) tu.hash = LSLAB_HASH_IDENTITY;
     if (tu.reclaim == 0) tu.reclaim = LSLAB_RECLAIM_OFF;
+    if (tu.spin_ms == 0) tu.spin_ms = (int)(SPIN_LIMIT_TICKS / TICKS_PER_MS);
 }
 
 static bool tuning_ok(const lslab_tuning& tu) {
@@ -148,6 +149,7 @@ static bool tuning_ok(const lslab_tuning& tu) {
     if (tu.combine != LSLAB_COMBINE_WAVE && tu.combine != LSLAB_COMBINE_BLOCK) return false;
     if (tu.hash != LSLAB_HASH_IDENTITY && tu.hash != LSLAB_HASH_MIX) return false;
     if (tu.reclaim != LSLAB_RECLAIM_OFF && tu.reclaim != LSLAB_RECLAIM_ON) return false;
+    if (tu.spin_ms < 10 || tu.spin_ms > 3600000) return false;
     return true;
 }
 
@@ -395,6 +397,7 @@ static int create_common(lslab_table** out, TableType type, bool is_map, int kb,
     t->dev.pool = pool;
     t->dev.mask = (uint32_t)(t->nbuckets - 1);
     t->dev.hash_mix = t->tun.hash == LSLAB_HASH_MIX;
+    t->dev.spin_ticks = (uint64_t)t->tun.spin_ms * TICKS_PER_MS;
     t->device_bytes = table_device_bytes(t);
     *out = t;
     return LSLAB_OK;
@@ -468,14 +471,19 @@ extern "C" int lslab_set_tuning(lslab_table* t, const lslab_tuning* tuning) {
     const bool resize = tu.max_subbatch != t->tun.max_subbatch;
     const lslab_tuning prev = t->tun;
     t->tun = tu;
+    t->dev.spin_ticks = (uint64_t)tu.spin_ms * TICKS_PER_MS;
     if (resize) {
         // The scratch may still be in use by queued calls of this table.
         if (hipDeviceSynchronize() != hipSuccess) {
             t->tun = prev;
+            t->dev.spin_ticks = (uint64_t)prev.spin_ms * TICKS_PER_MS;
             return LSLAB_ERR_HIP;
         }
         const int rc = alloc_scratch(t);
-        if (rc) t->tun = prev;  // the previous scratch stays, sized for the previous tuning
+        if (rc) {  // the previous scratch stays, sized for the previous tuning
+            t->tun = prev;
+            t->dev.spin_ticks = (uint64_t)prev.spin_ms * TICKS_PER_MS;
+        }
         else t->device_bytes = table_device_bytes(t);
         return rc;
     }

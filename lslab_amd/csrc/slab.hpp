@@ -37,9 +37,11 @@ constexpr uint64_t NOSLAB = ~0ull;
 // Internal opcode (never in a caller's op stream): unlink every empty pool
 // slab of the key's chain (reclaim policy), under the bucket's lock.
 constexpr uint8_t OP_COMPACT = 3;
-// 10 s at the 100 MHz constant clock: bound on every spin so that a lost
-// lock holder turns into a status code instead of a hung device.
+// 10 s at the 100 MHz constant clock: the default bound on every spin, so
+// that a lost lock holder or barrier arrival turns into a status code instead
+// of a hung device (a table's tuning may set another, TableDev::spin_ticks).
 constexpr uint64_t SPIN_LIMIT_TICKS = 1000000000ull;
+constexpr uint64_t TICKS_PER_MS = 100000ull;
 
 template <typename K_, typename V_, bool MAP>
 struct Layout {
@@ -117,6 +119,7 @@ struct TableDev {
                            // RUN_CF | chain entry for the first mutation of a chain
     uint64_t rc_cap;       // 4 * (ops a run may hold)
     uint32_t rk_mask;      // key-table capacity - 1 (at least 2 * ops a run may hold)
+    uint64_t spin_ticks;   // bound on every device spin (100 MHz ticks)
 };
 
 // Operation stream: per-op opcode (or a uniform one), keys and values with a

@@ -48,6 +48,9 @@ def test_invalid_arguments_rejected_without_device_work(lib):
     assert lib.lslab_map_create(ctypes.byref(h), 8, 8, 10, 10, ctypes.byref(bad)) == _lib.ERR_INVALID
     bad = _lib.Tuning(4, 1, 256, 0, 1 << 20, 2, 3)  # hash outside IDENTITY/MIX
     assert lib.lslab_map_create(ctypes.byref(h), 8, 8, 10, 10, ctypes.byref(bad)) == _lib.ERR_INVALID
+    for spin in (5, 3600001):  # spin bound outside 10 ms .. 1 h
+        bad = _lib.Tuning(4, 1, 256, 0, 1 << 20, 2, 0, 0, spin)
+        assert lib.lslab_map_create(ctypes.byref(h), 8, 8, 10, 10, ctypes.byref(bad)) == _lib.ERR_INVALID
     assert lib.lslab_destroy(None) == _lib.ERR_INVALID
     assert lib.lslab_map_get(None, None, None, 0, None) == _lib.ERR_INVALID
 

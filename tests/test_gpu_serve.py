@@ -135,3 +135,33 @@ def test_admission_by_headroom(lslab, oracle_mod):
     ref = Replay(True, 8, 8, 0, pool_slabs=40)
     ref.apply(np.ones(len(present), np.uint8), present, present + np.uint64(1000))
     assert m.stats()["slabs_used"] == ref.slabs_used == 2
+
+
+def test_idle_gaps_longer_than_the_spin_bound(lslab, oracle_mod):
+    """A table whose spin bound is 50 ms serves requests separated by idle gaps
+    of 400 and 500 ms.  While the scheduler waits for the next arrival, the
+    other blocks wait in the grid barrier; its bound restarts at every
+    heartbeat of the scheduler, so the loop ends with status 0 and every
+    result is the sequential replay's (the keys are distinct)."""
+    say("serve idle gaps")
+    rng = np.random.default_rng(21)
+    sizes = np.array([100, 200, 50, 300, 10, 40])
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    n = int(offs[-1])
+    keys = rng.permutation(1 << 16)[:n].astype(np.uint64) * np.uint64(0x9E3779B97F4A7C15)
+    ops = rng.choice([0, 1, 2], size=n, p=[0.3, 0.6, 0.1]).astype(np.uint8)
+    vals = rng.integers(1, 2**63, size=n, dtype=np.uint64)
+    arrival = np.array([0, 1000, 400000, 401000, 900000, 905000]) * TICKS_PER_US
+    m = lslab.Map(8, 8, 10, 1 << 12, lslab.tuning(engine=1, max_subbatch=1 << 12, spin_ms=50))
+    assert m.get_tuning().spin_ms == 50
+    res = m.serve(_t(offs), _t(arrival), _t(ops, np.uint8), _t(keys.view(np.int64)),
+                  _t(vals.view(np.int64)), max_tick_ops=1 << 12)
+    assert m.status() == 0
+    assert res["rejected"] == 0 and res["served_ops"] == n and res["ticks"] >= 3
+    assert res["elapsed_ticks"] >= 905000 * TICKS_PER_US
+    comp = res["complete_ticks"].cpu().numpy()
+    assert (comp >= arrival).all()
+    ref = Replay(True, 8, 8, 10, pool_slabs=1 << 12)
+    rf, rv, _ = ref.apply(ops, keys, vals)
+    np.testing.assert_array_equal(res["found"].cpu().numpy(), rf)
+    np.testing.assert_array_equal(res["value"].contiguous().cpu().numpy().view(np.uint64), rv)
