@@ -316,54 +316,145 @@ struct ServeState {
 
 constexpr uint32_t SERVE_REJECTED = 0xffffffffu;
 
+// Inclusive prefix sum of v over the threads of a block (up to
+// PERSIST_THREADS), and the block's total in *total.  All threads call it.
+__device__ __forceinline__ uint64_t block_scan(uint64_t v, uint64_t* total) {
+    __shared__ uint64_t wsum[PERSIST_THREADS / 64];
+    const int lane = (int)__lane_id();
+    const int w = (int)(threadIdx.x / 64);
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t lo = __shfl_up((uint32_t)v, d, 64);
+        const uint32_t hi = __shfl_up((uint32_t)(v >> 32), d, 64);
+        if (lane >= d) v += ((uint64_t)hi << 32) | lo;
+    }
+    if (lane == 63) wsum[w] = v;
+    __syncthreads();
+    uint64_t before = 0, all = 0;
+    const int nw = (int)(blockDim.x / 64);
+    for (int i = 0; i < nw; ++i) {
+        if (i < w) before += wsum[i];
+        all += wsum[i];
+    }
+    __syncthreads();  // wsum is reused by the next call
+    *total = all;
+    return v + before;
+}
+
+// The next tick's admission, by all threads of block 0.  Thread 0 owns the
+// loop state.  It first finds the tick's first request, alone: it waits for
+// the request's arrival (an idle loop), or rejects it when the pool could
+// never hold its reservation.  Then the block takes the following requests in
+// chunks of one request per thread.  Each thread loads its request's arrival,
+// size and reservation; prefix sums give each request's running op and slab
+// totals; the tick admits the longest prefix whose requests have arrived and
+// fit the op cap, the request cap and the pool headroom.  These are the
+// rules of a request-by-request admission in arrival order, with the clock
+// read once per chunk.
 __device__ inline void serve_schedule(const TableDev& t, const ServeDev& sv, ServeState* ss,
                                       BarState* bar) {
-    uint64_t r = ss->next_req;
-    const uint64_t used = ld_agent(t.alloc_ctr) - ld_agent(t.free_count);
-    uint64_t reserved = 0, ops = 0, n = 0, lo = r;
-    while (r < sv.nreq) {
-        const uint64_t el = now_ticks() - ss->t0;
-        if (sv.arrival[r] > el) {
-            if (n > 0) break;
-            // Idle: wait for the next arrival, however far away, with a
-            // heartbeat every millisecond so that the blocks waiting in the
-            // grid barrier meanwhile do not take the loop for stuck.
-            uint64_t last = now_ticks();
-            for (;;) {
-                const uint64_t now = now_ticks();
-                if (now - ss->t0 >= sv.arrival[r]) break;
-                if (now - last >= TICKS_PER_MS) {
-                    last = now;
-                    __hip_atomic_fetch_add(&bar->beat, 1u, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT);
+    __shared__ uint64_t st_r, st_lo, st_n, st_ops, st_res, st_used, st_el;
+    __shared__ uint64_t st_pos[PERSIST_THREADS / 64];
+    __shared__ uint64_t st_ps, st_pn;
+    __shared__ int st_go;
+    if (threadIdx.x == 0) {
+        uint64_t r = ss->next_req;
+        const uint64_t used = ld_agent(t.alloc_ctr) - ld_agent(t.free_count);
+        uint64_t lo = r, n = 0, ops = 0, reserved = 0;
+        while (r < sv.nreq) {
+            if (sv.arrival[r] > now_ticks() - ss->t0) {
+                // Idle: wait for the next arrival, however far away, with a
+                // heartbeat every millisecond so that the blocks waiting in
+                // the grid barrier meanwhile do not take the loop for stuck.
+                uint64_t last = now_ticks();
+                for (;;) {
+                    const uint64_t now = now_ticks();
+                    if (now - ss->t0 >= sv.arrival[r]) break;
+                    if (now - last >= TICKS_PER_MS) {
+                        last = now;
+                        __hip_atomic_fetch_add(&bar->beat, 1u, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                    __builtin_amdgcn_s_sleep(8);
                 }
-                __builtin_amdgcn_s_sleep(8);
             }
-            continue;
-        }
-        const uint64_t sz = sv.off[r + 1] - sv.off[r];
-        if (n > 0 && (ops + sz > sv.max_tick_ops || n >= sv.max_tick_reqs)) break;
-        const uint64_t need = sv.reserve ? sv.reserve[r] : sz;
-        if (used + reserved + need > t.pool) {
-            if (n > 0) break;
-            sv.tick_of[r] = SERVE_REJECTED;
-            sv.complete[r] = now_ticks() - ss->t0;
-            ss->rejected++;
+            const uint64_t sz = sv.off[r + 1] - sv.off[r];
+            const uint64_t need = sv.reserve ? sv.reserve[r] : sz;
+            if (used + need > t.pool) {  // can never fit: the pool only shrinks
+                sv.tick_of[r] = SERVE_REJECTED;
+                sv.complete[r] = now_ticks() - ss->t0;
+                ss->rejected++;
+                r++;
+                lo = r;
+                continue;
+            }
+            n = 1;
+            ops = sz;
+            reserved = need;
             r++;
-            lo = r;
-            continue;
+            break;
         }
-        reserved += need;
-        ops += sz;
-        n++;
-        r++;
+        st_r = r;
+        st_lo = lo;
+        st_n = n;
+        st_ops = ops;
+        st_res = reserved;
+        st_used = used;
+        st_go = n > 0 && r < sv.nreq;
     }
-    // Read by every block after the barrier: agent-scope stores.
-    st_coh(&ss->r_lo, lo);
-    st_coh(&ss->r_hi, r);
-    ss->next_req = r;
-    ss->served_ops += ops;
-    if (n == 0) st_coh(&ss->done, 1u);
+    __syncthreads();
+    while (st_go) {
+        const uint64_t r = st_r;
+        if (threadIdx.x == 0) st_el = now_ticks() - ss->t0;
+        __syncthreads();
+        const uint64_t i = r + threadIdx.x;
+        const bool valid = i < sv.nreq;
+        uint64_t sz = 0, need = 0;
+        bool arrived = false;
+        if (valid) {
+            sz = sv.off[i + 1] - sv.off[i];
+            need = sv.reserve ? sv.reserve[i] : sz;
+            arrived = sv.arrival[i] <= st_el;
+        }
+        uint64_t tot;
+        const uint64_t ps = block_scan(sz, &tot);
+        const uint64_t pn = block_scan(need, &tot);
+        const bool ok = valid && arrived && st_ops + ps <= sv.max_tick_ops &&
+                        st_n + threadIdx.x + 1 <= (uint64_t)sv.max_tick_reqs &&
+                        st_used + st_res + pn <= t.pool;
+        // the first request that fails: the tick admits the ones before it
+        uint64_t f = ok ? (uint64_t)blockDim.x : (uint64_t)threadIdx.x;
+        for (int d = 32; d > 0; d >>= 1) {
+            const uint32_t lo = __shfl_xor((uint32_t)f, d, 64);
+            f = min(f, (uint64_t)lo);
+        }
+        if (__lane_id() == 0) st_pos[threadIdx.x / 64] = f;
+        __syncthreads();
+        uint64_t k = blockDim.x;
+        for (int w = 0; w < (int)(blockDim.x / 64); ++w) k = min(k, st_pos[w]);
+        if (k > 0 && threadIdx.x == k - 1) {
+            st_ps = ps;
+            st_pn = pn;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            if (k > 0) {
+                st_n += k;
+                st_ops += st_ps;
+                st_res += st_pn;
+                st_r = r + k;
+            }
+            st_go = k == blockDim.x && st_r < sv.nreq;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        // Read by every block after the barrier: agent-scope stores.
+        st_coh(&ss->r_lo, st_lo);
+        st_coh(&ss->r_hi, st_r);
+        ss->next_req = st_r;
+        ss->served_ops += st_ops;
+        if (st_n == 0) st_coh(&ss->done, 1u);
+    }
 }
 
 template <class L, int G, bool BC, bool RC>
@@ -381,7 +472,7 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_serve(TableDev t, OpsDev o,
         __syncthreads();
     if (blockIdx.x == 0 && threadIdx.x == 0) ss->t0 = now_ticks();
     for (uint64_t tick = 0;; ++tick) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) serve_schedule(t, sv, ss, bar);
+        if (blockIdx.x == 0) serve_schedule(t, sv, ss, bar);  // block-uniform
         if (!grid_sync(bar, gridDim.x, t.spin_ticks, false)) goto fail;
         {
             if (ld_agent(&ss->done)) break;
