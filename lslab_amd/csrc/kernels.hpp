@@ -181,7 +181,7 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_batches(TableDev t, OpsDev 
     uint32_t* cc = clr_counters();
     RunOffs* ro = run_offs();
     RepEnt* rb = t.rep + block_rep_base_fixed(submax, ng, gpb, blockIdx.x);
-    const uint64_t clr_base = block_rep_base_fixed(run_cap, ng, gpb, blockIdx.x);
+    const uint64_t clr_base = run_clr_stride(run_cap, gridDim.x) * blockIdx.x;
     if (threadIdx.x == 0) {
         *rc = 0;
         cc[0] = 0;
@@ -199,6 +199,12 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_batches(TableDev t, OpsDev 
     int par = 0;         // table set of the next run
     bool pend = false;   // the previous run's phase W is still to do (set 1 - par)
     uint64_t plo = 0;    // that run's first op
+    // phase W of the pending run, in the other table set
+    auto flush_w = [&]() {
+        const RunTabs pt = par ? t.run[0] : t.run[1];
+        run_phase_w<L, G, RC>(t, pt, o, plo, rb, *rc, rep_lds(), REP_LDS, pt.clr + clr_base,
+                              cc[par ^ 1]);
+    };
     uint64_t b = 0;
     while (b < nbatches) {
         // The run [b, b + nb): one thread per boundary loads the window
@@ -217,9 +223,7 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_batches(TableDev t, OpsDev 
         if (nr <= 1) {  // batch b alone, in sub-batches of at most submax ops
             const uint64_t hi = ro->win[1];
             if (pend) {
-                const RunTabs pt = par ? t.run[0] : t.run[1];
-                run_phase_w<L, G, RC>(t, pt, o, plo, rb, *rc, rep_lds(), REP_LDS,
-                                      pt.clr + clr_base, cc[par ^ 1]);
+                flush_w();
                 if (!grid_sync(bar, gridDim.x, t.spin_ticks, true, rc, &cc[par ^ 1])) return fail();
                 pend = false;
             }
@@ -239,12 +243,8 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_batches(TableDev t, OpsDev 
         if (threadIdx.x <= nr) ro->off[threadIdx.x] = ro->win[threadIdx.x] - lo;
         __syncthreads();
         const RunTabs rt = par ? t.run[1] : t.run[0];
-        if (pend) {  // the previous run's write-back, in the other table set
-            const RunTabs pt = par ? t.run[0] : t.run[1];
-            run_phase_w<L, G, RC>(t, pt, o, plo, rb, *rc, rep_lds(), REP_LDS, pt.clr + clr_base,
-                                  cc[par ^ 1]);
-        }
-        run_pass_a<L, G>(t, rt, o, out, lo, m, *ro, gid, ng, rt.clr + clr_base, &cc[par]);
+        if (pend) flush_w();  // the previous run's write-back, in the other table set
+        run_pass_a<L>(t, rt, o, out, lo, m, *ro, rt.clr + clr_base, &cc[par]);
         // pass B reads the table (after that write-back) and the run tables
         // with plain loads: drop stale lines
         if (!grid_sync(bar, gridDim.x, t.spin_ticks, true, rc, &cc[par ^ 1])) return fail();
@@ -256,9 +256,7 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_batches(TableDev t, OpsDev 
         b += nr;
     }
     if (pend) {
-        const RunTabs pt = par ? t.run[0] : t.run[1];
-        run_phase_w<L, G, RC>(t, pt, o, plo, rb, *rc, rep_lds(), REP_LDS, pt.clr + clr_base,
-                              cc[par ^ 1]);
+        flush_w();
         if (!grid_sync(bar, gridDim.x, t.spin_ticks, true, rc, &cc[par ^ 1])) return fail();
     }
 }

@@ -57,9 +57,16 @@ constexpr uint64_t NA_REP_LEN = NA_SUB + NA_BLOCKS * PERSIST_THREADS;
 // two sets of run tables of the batched kernel (runs of at most NA_SUB ops)
 constexpr uint64_t NA_RK_LEN = NA_GT_CAP + 2;
 constexpr uint64_t NA_RC_LEN = 4 * NA_SUB + RUN_MAX_BATCHES;
+constexpr uint64_t na_clr_len() {
+    uint64_t len = 0;
+    for (uint64_t nb = 1; nb <= NA_BLOCKS; ++nb)
+        len = len > nb * run_clr_stride(NA_SUB, nb) ? len : nb * run_clr_stride(NA_SUB, nb);
+    return len;
+}
+constexpr uint64_t NA_CLR_LEN = na_clr_len();
 constexpr uint64_t NA_SET_BYTES = na_align(NA_RK_LEN * sizeof(RunKey)) +
                                   na_align(NA_RC_LEN * sizeof(RunChain)) +
-                                  na_align(NA_REP_LEN * sizeof(uint32_t));
+                                  na_align(NA_CLR_LEN * sizeof(uint32_t));
 constexpr uint64_t NA_SET_OFF = na_align(NA_REP_OFF + NA_REP_LEN * sizeof(RepEnt));
 constexpr uint64_t NA_PRED_OFF = NA_SET_OFF + 2 * NA_SET_BYTES;
 constexpr uint64_t NA_BYTES = na_align(NA_PRED_OFF + NA_SUB * sizeof(uint32_t));
@@ -185,15 +192,18 @@ static uint64_t rep_len(const lslab_table* t) {
 
 // Run tables of the batched kernel, two sets: key table (rk_cap + 2 entries),
 // chain tables (4 * run_cap + RUN_MAX_BATCHES entries) and per-block clear
-// lists (run_cap + one region slack per group of the persistent grid); and
-// the per-op predecessors (run_cap).
+// lists (as many entries as pass A can hand the blocks of any grid up to the
+// CU count: run_clr_stride); and the per-op predecessors (run_cap).
 static uint64_t rk_cap_of(uint64_t run_cap) {
     uint64_t cap = 1;
     while (cap < 2 * run_cap) cap <<= 1;
     return cap;
 }
 static uint64_t run_clr_len(const lslab_table* t, uint64_t run_cap) {
-    return run_cap + (uint64_t)t->num_cus * PERSIST_THREADS;
+    uint64_t len = 0;
+    for (uint64_t nb = 1; nb <= (uint64_t)t->num_cus; ++nb)
+        len = std::max(len, nb * run_clr_stride(run_cap, nb));
+    return len;
 }
 
 // Device memory the library holds for t (caller-provided slab memory excluded).

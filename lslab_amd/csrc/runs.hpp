@@ -161,108 +161,180 @@ __device__ __forceinline__ void corrupt(const TableDev& t) {
     __hip_atomic_fetch_or(t.status, ST_CORRUPT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Pass A over the run's ops [lo, lo + m), group gid of ng: the mutations.
-// Each wavefront chains its mutation leaders by (key, batch) in lane order;
-// the chain's last op exchanges on the (key, batch) entry.  A mutation whose
-// predecessor is known (an earlier op of its wavefront chain, or the chain
+// Ops of a run whose mutations one wavefront of pass A gathers per step:
+// 64 * RUN_A_CHUNKS at most, and fewer when the run is small, so that every
+// wavefront gets about one step.
+constexpr int RUN_A_CHUNKS = 16;
+
+// Chunks per pass-A step for a run of m ops over nw wavefronts.
+__host__ __device__ constexpr inline uint64_t run_a_chunks(uint64_t m, uint64_t nw) {
+    uint64_t p = (m + 64 * nw - 1) / (64 * nw);
+    return p > RUN_A_CHUNKS ? RUN_A_CHUNKS : (p ? p : 1);
+}
+
+// Ops pass A hands one block of nb (1024 threads each) for runs of at most
+// run_cap ops, and so the length of each block's clear list: a wavefront
+// takes ceil(steps / nw) steps of 64 * P ops.  It does not fall for a smaller
+// run (P shrinks only while every wavefront takes at most one step).
+__host__ __device__ constexpr inline uint64_t run_clr_stride(uint64_t run_cap, uint64_t nb) {
+    const uint64_t nw = nb * 16;
+    const uint64_t p = run_a_chunks(run_cap, nw);
+    const uint64_t steps = (run_cap + 64 * p - 1) / (64 * p);
+    return 1024 * p * ((steps + nw - 1) / nw);
+}
+
+// The wavefront's buffer of mutations gathered by pass A (op index | opcode
+// << 30), two rounds deep.
+__device__ __forceinline__ uint32_t* run_a_buf() {
+    __shared__ uint32_t buf[1024 / 64][128];
+    return buf[threadIdx.x / 64];
+}
+
+// One round of pass A: the wavefront's c <= 64 gathered mutations, one per
+// lane.  Mutations of one (key, batch) are chained in lane order; the chain's
+// last op exchanges on the (key, batch) entry of the chain table.  A mutation
+// whose predecessor is known (an earlier op of its chain, or the chain that
 // exchanged before) writes its result now; the first mutation of a (key,
 // batch) chain is left to pass B (run_pred = RUN_CF | entry).  The op that
 // claims a (key, batch) entry registers the batch in the key's bitmap and
-// lists the entry for clearing (clr, one list per block, counted by ccnt in
-// LDS), in parallel with its exchange.  The next op's record is loaded while
-// the current one's atomics are in flight.
-template <class L, int G>
-__device__ __forceinline__ void run_pass_a(const TableDev& t, const RunTabs& rt, const OpsDev& o,
-                                           const OutDev& out, uint64_t lo, uint64_t m,
-                                           const RunOffs& ro, uint64_t gid, uint64_t ng,
-                                           uint32_t* clr, uint32_t* ccnt) {
+// lists the entry for clearing (clr, counted by ccnt in LDS), in parallel
+// with its exchange.
+template <class L>
+__device__ __forceinline__ void run_a_round(const TableDev& t, const RunTabs& rt, const OpsDev& o,
+                                            const OutDev& out, uint64_t lo, uint64_t m,
+                                            const RunOffs& ro, const uint32_t* buf, uint32_t c,
+                                            uint32_t* clr, uint32_t* ccnt) {
     using K = typename L::K;
     using V = typename L::V;
-    const int gl = Grp<G>::lane();
     const int lane = (int)__lane_id();
-    uint8_t op_n = OP_GET;
-    K key_n = K(0);
-    if (gid < m) {
-        op_n = ld_opcode(o, lo + gid);
-        key_n = ld_stride<K>(o.keys, o.kstride, lo + gid);
+    const bool lead = (uint32_t)lane < c;
+    uint32_t li = 0;
+    uint8_t op = OP_GET;
+    K key = K(0);
+    int r = 0;
+    if (lead) {
+        const uint32_t e = buf[lane];
+        li = e & 0x3fffffffu;
+        op = (uint8_t)(e >> 30);
+        key = ld_stride<K>(o.keys, o.kstride, lo + li);
+        r = run_batch_of(ro, li);
     }
-    // The groups of a wavefront have consecutive ids, so the trip count is
-    // wave-uniform; the ballot keeps it so.
-    for (uint64_t li = gid; __ballot(li < m) != 0; li += ng) {
-        const bool act = li < m;
-        const uint8_t op = op_n;
-        const K key = key_n;
-        if (li + ng < m) {
-            op_n = ld_opcode(o, lo + li + ng);
-            key_n = ld_stride<K>(o.keys, o.kstride, lo + li + ng);
+    const uint64_t kk = (uint64_t)key;
+    uint64_t todo = __ballot(lead);
+    int my_last = lane, my_prev = -1;
+    while (todo) {
+        const int l = __builtin_ctzll(todo);
+        const uint64_t kl = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(kk >> 32), l) << 32) |
+                            (uint32_t)__builtin_amdgcn_readlane((uint32_t)kk, l);
+        const int rl = __builtin_amdgcn_readlane(r, l);
+        const bool mine = lead && kk == kl && r == rl;
+        const uint64_t same = __ballot(mine);
+        todo &= ~same;
+        if (mine) {
+            my_last = 63 - __builtin_clzll(same);
+            const uint64_t below = same & ((1ull << lane) - 1ull);
+            my_prev = below ? 63 - __builtin_clzll(below) : -1;
         }
-        const bool lead = act && op != OP_GET && gl == 0;
-        const int r = lead ? run_batch_of(ro, li) : 0;
-        const uint64_t kk = (uint64_t)key;
-        uint64_t todo = __ballot(lead);
-        if (todo == 0) continue;  // wave-uniform
-        int my_last = lane, my_prev = -1;
-        while (todo) {
-            const int l = __builtin_ctzll(todo);
-            const uint64_t kl = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(kk >> 32), l) << 32) |
-                                (uint32_t)__builtin_amdgcn_readlane((uint32_t)kk, l);
-            const int rl = __builtin_amdgcn_readlane(r, l);
-            const bool mine = lead && kk == kl && r == rl;
-            const uint64_t same = __ballot(mine);
-            todo &= ~same;
-            if (mine) {
-                my_last = 63 - __builtin_clzll(same);
-                const uint64_t below = same & ((1ull << lane) - 1ull);
-                my_prev = below ? 63 - __builtin_clzll(below) : -1;
+    }
+    const bool rep = lead && lane == my_last;
+    uint32_t ce = NOENT, fpred = 0;
+    bool won = false;
+    if (rep) {
+        uint64_t base;
+        uint32_t cmask;
+        run_region(ro, r, base, cmask);
+        ce = rc_claim(t, rt, kk, base, cmask, r, won);
+        if (ce != NOENT) {
+            fpred = __hip_atomic_exchange(&rt.rc[ce].head, li + 1u, __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+            if (kk == GT_EMPTY) won = fpred == 0;  // its dedicated entry's first exchange
+            if (won) {
+                const uint32_t gk = rk_claim(t, rt, kk);
+                if (gk != NOENT)
+                    __hip_atomic_fetch_or(&rt.rk[gk].mask, 1ull << r, __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+                else
+                    corrupt(t);
+            }
+        } else {
+            corrupt(t);
+        }
+    }
+    const uint32_t fp = __shfl(fpred, my_last, 64);
+    const uint32_t cc = __shfl(ce, my_last, 64);
+    const uint32_t prev_li = __shfl(li, my_prev >= 0 ? my_prev : lane, 64);
+    if (lead) {
+        const uint32_t pr = my_prev >= 0 ? prev_li + 1u : fp;
+        if (pr != 0) {
+            st_coh(&t.run_pred[li], pr);
+            mut_result<L>(t, o, out, lo, m, li, op, pr, false, V(0));
+        } else {
+            st_coh(&t.run_pred[li], RUN_CF | cc);
+        }
+    }
+    const uint64_t fm = __ballot(won);
+    if (fm) {
+        const int l0 = __builtin_ctzll(fm);
+        uint32_t wb = 0;
+        if (lane == l0)
+            wb = __hip_atomic_fetch_add(ccnt, (uint32_t)__builtin_popcountll(fm), __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_WORKGROUP);
+        wb = __builtin_amdgcn_readlane(wb, l0);
+        if (won)
+            st_coh(&clr[wb + (uint32_t)__builtin_popcountll(fm & ((1ull << lane) - 1ull))], ce);
+    }
+}
+
+// Pass A over the run's ops [lo, lo + m): the mutations.  Each wavefront
+// takes steps of 64 * P consecutive ops, reads them one 64-op chunk at a
+// time (one opcode per lane) and gathers the chunk's mutations into its LDS
+// buffer, so that a round of the chain-and-exchange work runs 64 mutations
+// at once however sparse they are among the gets.  P is chosen so that every
+// wavefront gets about one step.
+template <class L>
+__device__ __forceinline__ void run_pass_a(const TableDev& t, const RunTabs& rt, const OpsDev& o,
+                                           const OutDev& out, uint64_t lo, uint64_t m,
+                                           const RunOffs& ro, uint32_t* clr, uint32_t* ccnt) {
+    const int lane = (int)__lane_id();
+    const uint64_t nw = (uint64_t)gridDim.x * (blockDim.x / 64);
+    const uint64_t wid = (uint64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    const uint64_t P = run_a_chunks(m, nw);
+    uint32_t* buf = run_a_buf();
+    uint32_t cnt = 0;  // gathered, wave-uniform
+    for (uint64_t sb = wid * 64 * P; sb < m; sb += nw * 64 * P) {
+        // the step's opcodes, all loads in flight together
+        uint8_t ops[RUN_A_CHUNKS];
+#pragma unroll
+        for (int k = 0; k < RUN_A_CHUNKS; ++k) {
+            const uint64_t li = sb + 64 * k + lane;
+            ops[k] = (k < (int)P && li < m) ? ld_opcode(o, lo + li) : (uint8_t)OP_GET;
+        }
+#pragma unroll
+        for (int k = 0; k < RUN_A_CHUNKS; ++k) {
+            if (k >= (int)P) break;
+            const uint64_t li = sb + 64 * k + lane;
+            const uint8_t op = ops[k];
+            const bool mut = op != OP_GET;
+            const uint64_t bal = __ballot(mut);
+            if (mut)
+                buf[cnt + (uint32_t)__builtin_popcountll(bal & ((1ull << lane) - 1ull))] =
+                    (uint32_t)li | ((uint32_t)op << 30);
+            cnt += (uint32_t)__builtin_popcountll(bal);
+            if (cnt >= 64) {
+                __builtin_amdgcn_wave_barrier();
+                run_a_round<L>(t, rt, o, out, lo, m, ro, buf, 64, clr, ccnt);
+                // keep the overflow for the next round
+                const uint32_t carry = lane + 64 < (int)cnt ? buf[lane + 64] : 0u;
+                __builtin_amdgcn_wave_barrier();
+                if (lane + 64 < (int)cnt) buf[lane] = carry;
+                __builtin_amdgcn_wave_barrier();
+                cnt -= 64;
             }
         }
-        const bool rep = lead && lane == my_last;
-        uint32_t c = NOENT, fpred = 0;
-        bool won = false;
-        if (rep) {
-            uint64_t base;
-            uint32_t cmask;
-            run_region(ro, r, base, cmask);
-            c = rc_claim(t, rt, kk, base, cmask, r, won);
-            if (c != NOENT) {
-                fpred = __hip_atomic_exchange(&rt.rc[c].head, (uint32_t)li + 1u, __ATOMIC_RELAXED,
-                                              __HIP_MEMORY_SCOPE_AGENT);
-                if (kk == GT_EMPTY) won = fpred == 0;  // its dedicated entry's first exchange
-                if (won) {
-                    const uint32_t gk = rk_claim(t, rt, kk);
-                    if (gk != NOENT)
-                        __hip_atomic_fetch_or(&rt.rk[gk].mask, 1ull << r, __ATOMIC_RELAXED,
-                                              __HIP_MEMORY_SCOPE_AGENT);
-                    else
-                        corrupt(t);
-                }
-            } else {
-                corrupt(t);
-            }
-        }
-        const uint32_t fp = __shfl(fpred, my_last, 64);
-        const uint32_t cc = __shfl(c, my_last, 64);
-        const uint32_t prev_li = __shfl((uint32_t)li, my_prev >= 0 ? my_prev : lane, 64);
-        if (lead) {
-            const uint32_t pr = my_prev >= 0 ? prev_li + 1u : fp;
-            if (pr != 0) {
-                st_coh(&t.run_pred[li], pr);
-                mut_result<L>(t, o, out, lo, m, (uint32_t)li, op, pr, false, V(0));
-            } else {
-                st_coh(&t.run_pred[li], RUN_CF | cc);
-            }
-        }
-        const uint64_t fm = __ballot(won);
-        if (fm) {
-            const int l0 = __builtin_ctzll(fm);
-            uint32_t wb = 0;
-            if (lane == l0)
-                wb = __hip_atomic_fetch_add(ccnt, (uint32_t)__builtin_popcountll(fm),
-                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            wb = __builtin_amdgcn_readlane(wb, l0);
-            if (won)
-                st_coh(&clr[wb + (uint32_t)__builtin_popcountll(fm & ((1ull << lane) - 1ull))], c);
-        }
+    }
+    if (cnt) {
+        __builtin_amdgcn_wave_barrier();
+        run_a_round<L>(t, rt, o, out, lo, m, ro, buf, cnt, clr, ccnt);
     }
 }
 
