@@ -82,6 +82,20 @@ extern "C" {
 #define LSLAB_RECLAIM_OFF 1
 #define LSLAB_RECLAIM_ON 2
 
+/* Chain heads per key slot (new).  The two-phase engine orders each key's
+ * mutations of a sub-batch by one atomic exchange on that key's chain head.
+ * With ON the table keeps one 4-B head per key slot of its slab array
+ * (128 B per slab: +25% on a u64 map's 512-B slabs), and a key already in
+ * the table heads its chain at its own slot; otherwise (OFF, and keys not in
+ * the table) the head is an entry of the batch combining table, claimed by
+ * one compare-and-swap first.  One device round trip fewer per mutation:
+ * +5.8% on the 95/5 get/put Zipf-0.99 stream.  0 = AUTO: ON while the
+ * tuning's engine is TWO_PHASE.  create and set_tuning allocate or free the
+ * heads (LSLAB_ERR_OOM when they do not fit); a table of more than 2^26
+ * slabs has none. */
+#define LSLAB_SLOT_HEADS_ON 1
+#define LSLAB_SLOT_HEADS_OFF 2
+
 typedef struct lslab_table lslab_table;
 
 typedef struct {
@@ -99,6 +113,7 @@ typedef struct {
                               in ms, 10..3600000 (0 = 10000): a spin that makes no progress
                               for that long ends its kernel with LSLAB_ERR_TIMEOUT.  The
                               served loop's wait for an arrival is not bounded by it. */
+    int slot_heads;        /* LSLAB_SLOT_HEADS_* (0 = AUTO: ON while engine is TWO_PHASE) */
 } lslab_tuning;
 
 typedef struct {

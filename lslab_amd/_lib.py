@@ -31,6 +31,8 @@ HASH_MIX = 2
 
 RECLAIM_OFF = 1
 RECLAIM_ON = 2
+SLOT_HEADS_ON = 1
+SLOT_HEADS_OFF = 2
 
 CHAIN_BINS = 16
 
@@ -66,6 +68,7 @@ class Tuning(ctypes.Structure):
         ("hash", ctypes.c_int),
         ("reclaim", ctypes.c_int),
         ("spin_ms", ctypes.c_int),
+        ("slot_heads", ctypes.c_int),
     ]
 
 

@@ -523,6 +523,24 @@ __device__ __forceinline__ uint32_t gt_exchange(const TableDev& t, uint64_t key,
     return g;
 }
 
+// The global step of a chain whose key the probe found (or not) at slab s,
+// slot: a key in the table heads its chain at its own slot when the table
+// keeps per-slot heads (no claim), any other key at its combining-table
+// entry.  Returns the head's id for phase W (LOC_G | slot index, or the
+// entry).
+template <bool CAS_FIRST = false>
+__device__ __forceinline__ uint32_t chain_exchange(const TableDev& t, uint64_t key, bool found,
+                                                   uint64_t s, int slot, uint32_t v,
+                                                   uint32_t& pred) {
+    if (t.loc_head && found) {
+        const uint32_t x = (uint32_t)((s << 5) | (uint64_t)slot);
+        pred = __hip_atomic_exchange(&t.loc_head[x], v, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT);
+        return LOC_G | x;
+    }
+    return gt_exchange<CAS_FIRST>(t, key, v, pred);
+}
+
 // Writes the result of mutation (lo + li) with opcode op, whose serial
 // predecessor in the sub-batch (or run) is op (lo + pred - 1), or, for
 // pred == 0, the key's pre-batch state (found, cur).
@@ -637,14 +655,14 @@ __device__ __forceinline__ void phase_r_op(const TableDev& t, const OpsDev& o, c
                 ct->key[j] = GT_EMPTY;
                 ct->head[j] = 0;
                 uint32_t P;
-                g = gt_exchange(t, kk, li + 1u, P);
+                g = chain_exchange(t, kk, found, s, slot, li + 1u, P);
                 uint8_t fop = OP_PUT;
                 if constexpr (!L::IS_MAP) fop = ld_opcode(o, lo + fl);
                 mut_result<L>(t, o, out, lo, m, fl, fop, P, found, cur);
             }
         }
     } else if (rep) {
-        g = gt_exchange<true>(t, kk, li + 1u, fpred);
+        g = chain_exchange<true>(t, kk, found, s, slot, li + 1u, fpred);
     }
     fpred = __shfl(fpred, my_last, 64);
     if constexpr (BC) fdefer = __shfl((int)fdefer, my_last, 64) != 0;
@@ -966,9 +984,10 @@ __device__ __forceinline__ void phase_w_ent(const TableDev& t, const OpsDev& o, 
     const uint8_t op = ld_opcode(o, i);
     const K key = ld_stride<K>(o.keys, o.kstride, i);
     const V val = (L::IS_MAP && o.vals) ? ld_stride<V>(o.vals, o.vstride, i) : V(0);
-    if (ld_agent(&t.gt_head[e.g]) != e.li + 1u) return;  // a later mutation of this key exists
+    uint32_t* head = (e.g & LOC_G) ? &t.loc_head[e.g & ~LOC_G] : &t.gt_head[e.g];
+    if (ld_agent(head) != e.li + 1u) return;  // a later mutation of this key exists
     if (gl == 0) {
-        __hip_atomic_store(&t.gt_head[e.g], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(head, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (e.g <= t.gt_mask)
             __hip_atomic_store(&t.gt_key[e.g], GT_EMPTY, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }

@@ -157,7 +157,42 @@ static bool tuning_ok(const lslab_tuning& tu) {
     if (tu.hash != LSLAB_HASH_IDENTITY && tu.hash != LSLAB_HASH_MIX) return false;
     if (tu.reclaim != LSLAB_RECLAIM_OFF && tu.reclaim != LSLAB_RECLAIM_ON) return false;
     if (tu.spin_ms < 10 || tu.spin_ms > 3600000) return false;
+    if (tu.slot_heads != 0 && tu.slot_heads != LSLAB_SLOT_HEADS_ON &&
+        tu.slot_heads != LSLAB_SLOT_HEADS_OFF)
+        return false;
     return true;
+}
+
+static uint64_t slot_heads_len(const lslab_table* t) { return (t->nbuckets + t->pool) * 32; }
+
+// Whether the tuning asks for per-slot chain heads (and the slot indices fit
+// below LOC_G).
+static bool slot_heads_wanted(const lslab_table* t, const lslab_tuning& tu) {
+    const bool on = tu.slot_heads == LSLAB_SLOT_HEADS_ON ||
+                    (tu.slot_heads == 0 && tu.engine == LSLAB_ENGINE_TWO_PHASE);
+    return on && slot_heads_len(t) <= LOC_G;
+}
+
+// Allocates (zeroed) or frees the per-slot chain heads as the tuning asks.
+// Freeing waits for the device: queued calls may still use them.
+static int sync_slot_heads(lslab_table* t) {
+    const bool want = slot_heads_wanted(t, t->tun);
+    if (want && !t->dev.loc_head) {
+        uint32_t* h = nullptr;
+        if (hipMalloc(&h, slot_heads_len(t) * sizeof(uint32_t)) != hipSuccess)
+            return LSLAB_ERR_OOM;
+        if (hipMemset(h, 0, slot_heads_len(t) * sizeof(uint32_t)) != hipSuccess ||
+            hipDeviceSynchronize() != hipSuccess) {
+            (void)hipFree(h);
+            return LSLAB_ERR_HIP;
+        }
+        t->dev.loc_head = h;
+    } else if (!want && t->dev.loc_head) {
+        if (hipDeviceSynchronize() != hipSuccess) return LSLAB_ERR_HIP;
+        (void)hipFree(t->dev.loc_head);
+        t->dev.loc_head = nullptr;
+    }
+    return LSLAB_OK;
 }
 
 static void free_table(lslab_table* t) {
@@ -175,6 +210,7 @@ static void free_table(lslab_table* t) {
         (void)hipFree(r.clr);
     }
     (void)hipFree(t->dev.run_pred);
+    (void)hipFree(t->dev.loc_head);
     if (t->order_ev) (void)hipEventDestroy(t->order_ev);
     delete t;
 }
@@ -213,7 +249,8 @@ static uint64_t table_device_bytes(const lslab_table* t) {
            rep_cnt_len(t->submax) * 4 + t->run_cap * 4 +
            2 * ((rk_cap_of(t->run_cap) + 2) * sizeof(RunKey) +
                 (4 * t->run_cap + RUN_MAX_BATCHES) * sizeof(RunChain) +
-                run_clr_len(t, t->run_cap) * 4);
+                run_clr_len(t, t->run_cap) * 4) +
+           (t->dev.loc_head ? slot_heads_len(t) * 4 : 0);
 }
 
 // Empties run tables (setup only; the kernels leave them empty after every run).
@@ -390,6 +427,7 @@ static int create_common(lslab_table** out, TableType type, bool is_map, int kb,
         return LSLAB_ERR_HIP;
     }
     rc = alloc_scratch(t);
+    if (rc == LSLAB_OK) rc = sync_slot_heads(t);
     if (rc != LSLAB_OK) {
         free_table(t);
         return rc;
@@ -480,24 +518,23 @@ extern "C" int lslab_set_tuning(lslab_table* t, const lslab_tuning* tuning) {
     if (tu.reclaim != t->tun.reclaim) return LSLAB_ERR_INVALID;  // fixed with the table
     const bool resize = tu.max_subbatch != t->tun.max_subbatch;
     const lslab_tuning prev = t->tun;
+    auto restore = [&]() {
+        t->tun = prev;
+        t->dev.spin_ticks = (uint64_t)prev.spin_ms * TICKS_PER_MS;
+    };
     t->tun = tu;
     t->dev.spin_ticks = (uint64_t)tu.spin_ms * TICKS_PER_MS;
-    if (resize) {
+    int rc = sync_slot_heads(t);
+    if (rc == LSLAB_OK && resize) {
         // The scratch may still be in use by queued calls of this table.
-        if (hipDeviceSynchronize() != hipSuccess) {
-            t->tun = prev;
-            t->dev.spin_ticks = (uint64_t)prev.spin_ms * TICKS_PER_MS;
-            return LSLAB_ERR_HIP;
-        }
-        const int rc = alloc_scratch(t);
-        if (rc) {  // the previous scratch stays, sized for the previous tuning
-            t->tun = prev;
-            t->dev.spin_ticks = (uint64_t)prev.spin_ms * TICKS_PER_MS;
-        }
-        else t->device_bytes = table_device_bytes(t);
-        return rc;
+        rc = hipDeviceSynchronize() == hipSuccess ? alloc_scratch(t) : LSLAB_ERR_HIP;
     }
-    return LSLAB_OK;
+    if (rc) {  // the previous scratch stays, sized for the previous tuning
+        restore();
+        (void)sync_slot_heads(t);  // heads as the previous tuning had them
+    }
+    t->device_bytes = table_device_bytes(t);
+    return rc;
 }
 
 extern "C" int lslab_get_tuning(const lslab_table* t, lslab_tuning* out) {

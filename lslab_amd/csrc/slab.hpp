@@ -120,7 +120,13 @@ struct TableDev {
     uint64_t rc_cap;       // 4 * (ops a run may hold)
     uint32_t rk_mask;      // key-table capacity - 1 (at least 2 * ops a run may hold)
     uint64_t spin_ticks;   // bound on every device spin (100 MHz ticks)
+    uint32_t* loc_head;    // per key slot of the table (slab << 5 | slot): the last
+                           // mutation (op index + 1) of the key found there in the
+                           // current sub-batch; null when not allocated
 };
+
+// RepEnt.g of a final-mutation candidate whose chain head is loc_head[g & ~LOC_G].
+constexpr uint32_t LOC_G = 0x80000000u;
 
 // Operation stream: per-op opcode (or a uniform one), keys and values with a
 // byte stride so that both SoA arrays and the reference's AoS pair<K,V> work.

@@ -51,6 +51,8 @@ def test_invalid_arguments_rejected_without_device_work(lib):
     for spin in (5, 3600001):  # spin bound outside 10 ms .. 1 h
         bad = _lib.Tuning(4, 1, 256, 0, 1 << 20, 2, 0, 0, spin)
         assert lib.lslab_map_create(ctypes.byref(h), 8, 8, 10, 10, ctypes.byref(bad)) == _lib.ERR_INVALID
+    bad = _lib.Tuning(4, 1, 256, 0, 1 << 20, 2, 0, 0, 0, 3)  # slot heads outside AUTO/ON/OFF
+    assert lib.lslab_map_create(ctypes.byref(h), 8, 8, 10, 10, ctypes.byref(bad)) == _lib.ERR_INVALID
     assert lib.lslab_destroy(None) == _lib.ERR_INVALID
     assert lib.lslab_map_get(None, None, None, 0, None) == _lib.ERR_INVALID
 
@@ -59,13 +61,16 @@ def test_cpp_headers_compile():
     """The header-only C++ API compiles on its own (host-only translation unit)."""
     import subprocess
     import tempfile
-    src = ('#include <lslab/map.h>\n#include <lslab/set.h>\n#include <lslab/wide.h>\n'
+    src = ('#include <cstddef>\n#include <lslab/map.h>\n#include <lslab/set.h>\n'
+           '#include <lslab/wide.h>\n'
            'struct K20 { unsigned w[5]; };\n'
            'template class lslab::wide_map<K20, unsigned long long>;\n'
            'template class lslab::wide_set<K20>;\n'
            'int main(){ static_assert(sizeof(lslab::found_value<unsigned long long>)==16,"");'
            ' static_assert(sizeof(lslab::found_value<unsigned>)==8,"");'
            ' static_assert(sizeof(lslab::key_value<unsigned,unsigned long long>)==16,"");'
+           f' static_assert(sizeof(lslab_tuning)=={ctypes.sizeof(_lib.Tuning)},"");'
+           f' static_assert(offsetof(lslab_tuning, slot_heads)=={_lib.Tuning.slot_heads.offset},"");'
            ' return 0; }\n')
     with tempfile.TemporaryDirectory() as d:
         f = os.path.join(d, "t.cpp")

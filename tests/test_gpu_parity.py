@@ -42,8 +42,10 @@ def to_host(t, b):
     return a.view(np.uint32).astype(np.uint64) if b == 4 else a.view(np.uint64)
 
 
-def make(lslab, is_map, kb, vb, n_log2, pool, engine=0, lanes=16, subbatch=1 << 16, combine=2):
-    tun = lslab.tuning(lanes_per_op=lanes, engine=engine, max_subbatch=subbatch, combine=combine)
+def make(lslab, is_map, kb, vb, n_log2, pool, engine=0, lanes=16, subbatch=1 << 16, combine=2,
+         slot_heads=0):
+    tun = lslab.tuning(lanes_per_op=lanes, engine=engine, max_subbatch=subbatch, combine=combine,
+                       slot_heads=slot_heads)
     if is_map:
         return lslab.Map(kb, vb, n_log2, pool, tun)
     return lslab.Set(kb, n_log2, pool, tun)
@@ -178,21 +180,25 @@ def test_phase_separated_bit_exact(lslab, oracle_mod, is_map, kb, vb, engine, la
     assert st["max_chain"] >= 2
 
 
-# (engine, lanes per op, combining scope): the plain engine, and the two-phase
-# engine with wavefront-only and block-wide combining of same-key mutations.
-CONFLICT_MODES = [(0, 16, 2), (1, 16, 1), (1, 16, 2), (1, 1, 1), (1, 1, 2), (1, 4, 2)]
+# (engine, lanes per op, combining scope, slot heads): the plain engine, and
+# the two-phase engine with wavefront-only and block-wide combining of
+# same-key mutations, chained at per-slot heads (0: on with the two-phase
+# engine) or at combining-table entries only (2: off).
+CONFLICT_MODES = [(0, 16, 2, 0), (1, 16, 1, 0), (1, 16, 2, 0), (1, 1, 1, 0), (1, 1, 2, 0),
+                  (1, 4, 2, 0), (1, 16, 1, 2), (1, 1, 1, 2), (1, 4, 2, 2)]
 
 
 @pytest.mark.parametrize("is_map,kb,vb", [(True, 8, 8), (True, 4, 4), (False, 8, 0)])
-@pytest.mark.parametrize("engine,lanes,combine", CONFLICT_MODES)
+@pytest.mark.parametrize("engine,lanes,combine,heads", CONFLICT_MODES)
 def test_conflicting_batches_serial_order(lslab, oracle_mod, is_map, kb, vb, engine, lanes,
-                                          combine):
+                                          combine, heads):
     """Zipf-like hot keys with get/put/remove conflicts inside each batch:
     every key's results must match some serial order of its ops."""
-    say("conflicting", is_map, kb, vb, engine, lanes, combine)
+    say("conflicting", is_map, kb, vb, engine, lanes, combine, heads)
     rng = np.random.default_rng(5 + engine)
     n_log2 = 5
-    tbl = make(lslab, is_map, kb, vb, n_log2, 1 << 14, engine, lanes, combine=combine)
+    tbl = make(lslab, is_map, kb, vb, n_log2, 1 << 14, engine, lanes, combine=combine,
+               slot_heads=heads)
     ref = Replay(is_map, kb, vb, n_log2, pool_slabs=1 << 14)
     space = 3000
     p = 1.0 / np.arange(1, space + 1) ** 1.1
@@ -712,3 +718,37 @@ def test_set_tuning_resizes_scratch(lslab, oracle_mod):
             assert tbl.stats()["device_bytes"] > before
     assert tbl.stats()["device_bytes"] == before
     assert tbl.status() == 0
+
+
+def test_slot_heads_follow_tuning(lslab, oracle_mod):
+    """Per-slot chain heads are allocated while the tuning asks for them (AUTO:
+    with the two-phase engine) and freed when it stops; device_bytes follows
+    (4 B per key slot), and results stay the replay's in every mode, on a
+    stream that mutates present and absent keys."""
+    say("slot_heads_follow_tuning")
+    n_log2, pool = 6, 1 << 10
+    tbl = make(lslab, True, 8, 8, n_log2, pool, 0, 4, subbatch=1 << 14)
+    ref = Replay(True, 8, 8, n_log2, pool_slabs=pool)
+    heads = ((1 << n_log2) + pool) * 32 * 4
+    b0 = tbl.stats()["device_bytes"]
+    rng = np.random.default_rng(21)
+    modes = [(1, 0, heads), (1, lslab.SLOT_HEADS_OFF, 0), (0, lslab.SLOT_HEADS_ON, heads),
+             (0, 0, 0), (1, lslab.SLOT_HEADS_ON, heads)]
+    for engine, sh, extra in modes:
+        tbl.set_tuning(lslab.tuning(lanes_per_op=4, engine=engine, max_subbatch=1 << 14,
+                                    combine=lslab.COMBINE_WAVE, slot_heads=sh))
+        assert tbl.stats()["device_bytes"] == b0 + extra, (engine, sh)
+        for _ in range(3):
+            keys = rng.integers(1, 4000, size=6000, dtype=np.uint64)
+            ops = rng.choice([OP_GET, OP_PUT, OP_REMOVE], size=len(keys),
+                             p=[0.5, 0.35, 0.15]).astype(np.uint8)
+            vals = rng.integers(1, 2**63, size=len(keys), dtype=np.uint64)
+            f, v = run_batch(tbl, True, 8, 8, ops, keys, vals)
+            pf, pv = run_batch(tbl, True, 8, 8, np.zeros(len(keys), np.uint8), keys, vals)
+            bad, first = ref.check_batch(ops, keys, vals, f, v, pf, pv)
+            assert bad == 0, (engine, sh, first)
+    assert tbl.status() == 0
+    k, vv = dump_sorted(tbl, True, 8, 8)
+    rk, rv = oracle_dump_sorted(ref)
+    np.testing.assert_array_equal(k, rk)
+    np.testing.assert_array_equal(vv, rv)
