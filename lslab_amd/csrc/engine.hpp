@@ -601,23 +601,17 @@ __device__ __forceinline__ void phase_r_op(const TableDev& t, const OpsDev& o, c
     if constexpr (L::IS_MAP) {
         if (found) cur = *val_slot<L>(t, s, slot);
     }
-    if constexpr (BC) {
-        if (act && op == OP_GET && gl == 0) write_result<L>(out, lo + li, found, cur);
-    } else {  // no block barrier ahead: gets are done
-        if (op == OP_GET) {
-            if (gl == 0) write_result<L>(out, lo + li, found, cur);
-            return;
-        }
-    }
+    // The value load stays in flight while the wavefront chains its mutations
+    // and issues their exchanges; gets write their results after that.
+    const bool wget = act && op == OP_GET && gl == 0;
     const int lane = (int)__lane_id();
     const bool lead = act && op != OP_GET && gl == 0;
     const uint64_t kk = (uint64_t)key;
     uint64_t todo = __ballot(lead);
-    if constexpr (BC) {
-        if (todo == 0) {  // wave-uniform: no mutation in this wavefront
-            lds_barrier();
-            return;
-        }
+    if (todo == 0) {  // wave-uniform: no mutation in this wavefront
+        if (wget) write_result<L>(out, lo + li, found, cur);
+        if constexpr (BC) lds_barrier();
+        return;
     }
     // Wave chains: one pass per distinct key among the mutation leaders.
     int my_first = lane, my_last = lane, my_prev = -1;
@@ -664,6 +658,7 @@ __device__ __forceinline__ void phase_r_op(const TableDev& t, const OpsDev& o, c
     } else if (rep) {
         g = chain_exchange<true>(t, kk, found, s, slot, li + 1u, fpred);
     }
+    if (wget) write_result<L>(out, lo + li, found, cur);
     fpred = __shfl(fpred, my_last, 64);
     if constexpr (BC) fdefer = __shfl((int)fdefer, my_last, 64) != 0;
     const uint32_t prev_li = __shfl(li, my_prev >= 0 ? my_prev : lane, 64);
