@@ -38,7 +38,8 @@ struct BarState {
     BarWord group[BAR_GROUPS];
     uint32_t abort;
     uint32_t beat;  // heartbeat of a live wait elsewhere (the served loop's idle scheduler)
-    uint32_t pad[30];
+    uint32_t runs;  // batched launch: 1 if two consecutive batches fit one run (k_reset_batches)
+    uint32_t pad[29];
 };
 
 // Plain per-batch kernel: one G-lane group per op, grid-stride.  RC: the
@@ -169,11 +170,20 @@ __device__ __forceinline__ bool grid_sync(BarState* bar, uint32_t nblocks, uint6
 // A batch that forms no run with its successor runs alone after any pending
 // phase W: phase R, barrier, phase W, barrier, in sub-batches of submax ops
 // when it is larger.
-template <class L, int G, bool BC, bool RC>
+//
+// RUNS = false is the same kernel without the run path: every batch runs
+// alone.  A stream in which no two consecutive batches fit one run (the
+// launcher's k_reset_batches finds out, bar->runs = 0) takes that kernel,
+// whose phase R loop keeps all of its state in registers (the run path's
+// code raises the whole kernel's register use past the 128 a 1024-thread
+// block allows, and the compiler then spills).  Both kernels are launched;
+// the one that does not match bar->runs returns at once.
+template <class L, int G, bool BC, bool RC, bool RUNS>
 __global__ __launch_bounds__(PERSIST_THREADS) void k_batches(TableDev t, OpsDev o, OutDev out,
                                                  const uint64_t* offsets, uint64_t nbatches,
                                                  uint64_t submax, uint64_t run_cap,
                                                  BarState* bar) {
+    if ((ld_agent(&bar->runs) != 0) != RUNS) return;  // grid-uniform
     const uint64_t ng = (uint64_t)gridDim.x * blockDim.x / G;
     const uint64_t gid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
     const uint64_t gpb = blockDim.x / G;
@@ -209,7 +219,7 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_batches(TableDev t, OpsDev 
     while (b < nbatches) {
         // The run [b, b + nb): one thread per boundary loads the window
         // offsets[b .. b + RUN_MAX_BATCHES] into LDS, one thread scans it.
-        const uint64_t wn = min((uint64_t)RUN_MAX_BATCHES, nbatches - b);
+        const uint64_t wn = min(RUNS ? (uint64_t)RUN_MAX_BATCHES : 1, nbatches - b);
         if (threadIdx.x <= wn) ro->win[threadIdx.x] = offsets[b + threadIdx.x];
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -219,13 +229,16 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_batches(TableDev t, OpsDev 
         }
         __syncthreads();
         const uint64_t lo = ro->win[0];
-        const uint64_t nr = (uint64_t)ro->nb;
+        const uint64_t nr = RUNS ? (uint64_t)ro->nb : 1;
         if (nr <= 1) {  // batch b alone, in sub-batches of at most submax ops
             const uint64_t hi = ro->win[1];
-            if (pend) {
-                flush_w();
-                if (!grid_sync(bar, gridDim.x, t.spin_ticks, true, rc, &cc[par ^ 1])) return fail();
-                pend = false;
+            if constexpr (RUNS) {
+                if (pend) {
+                    flush_w();
+                    if (!grid_sync(bar, gridDim.x, t.spin_ticks, true, rc, &cc[par ^ 1]))
+                        return fail();
+                    pend = false;
+                }
             }
             for (uint64_t slo = lo; slo < hi; slo += submax) {
                 const uint64_t m = min(submax, hi - slo);
@@ -239,25 +252,29 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_batches(TableDev t, OpsDev 
             b = b + 1;
             continue;
         }
-        const uint64_t m = ro->win[nr] - lo;
-        if (threadIdx.x <= nr) ro->off[threadIdx.x] = ro->win[threadIdx.x] - lo;
-        __syncthreads();
-        const RunTabs rt = par ? t.run[1] : t.run[0];
-        if (pend) flush_w();  // the previous run's write-back, in the other table set
-        run_pass_a<L>(t, rt, o, out, lo, m, *ro, rt.clr + clr_base, &cc[par]);
-        // pass B reads the table (after that write-back) and the run tables
-        // with plain loads: drop stale lines
-        if (!grid_sync(bar, gridDim.x, t.spin_ticks, true, rc, &cc[par ^ 1])) return fail();
-        run_pass_b<L, G>(t, rt, o, out, lo, m, *ro, gid, ng, rb, rep_lds(), REP_LDS, rc);
-        if (!grid_sync(bar, gridDim.x, t.spin_ticks, false)) return fail();
-        pend = true;
-        plo = lo;
-        par ^= 1;
-        b += nr;
+        if constexpr (RUNS) {
+            const uint64_t m = ro->win[nr] - lo;
+            if (threadIdx.x <= nr) ro->off[threadIdx.x] = ro->win[threadIdx.x] - lo;
+            __syncthreads();
+            const RunTabs rt = par ? t.run[1] : t.run[0];
+            if (pend) flush_w();  // the previous run's write-back, in the other table set
+            run_pass_a<L>(t, rt, o, out, lo, m, *ro, rt.clr + clr_base, &cc[par]);
+            // pass B reads the table (after that write-back) and the run tables
+            // with plain loads: drop stale lines
+            if (!grid_sync(bar, gridDim.x, t.spin_ticks, true, rc, &cc[par ^ 1])) return fail();
+            run_pass_b<L, G>(t, rt, o, out, lo, m, *ro, gid, ng, rb, rep_lds(), REP_LDS, rc);
+            if (!grid_sync(bar, gridDim.x, t.spin_ticks, false)) return fail();
+            pend = true;
+            plo = lo;
+            par ^= 1;
+            b += nr;
+        }
     }
-    if (pend) {
-        flush_w();
-        if (!grid_sync(bar, gridDim.x, t.spin_ticks, true, rc, &cc[par ^ 1])) return fail();
+    if constexpr (RUNS) {
+        if (pend) {
+            flush_w();
+            if (!grid_sync(bar, gridDim.x, t.spin_ticks, true, rc, &cc[par ^ 1])) return fail();
+        }
     }
 }
 
@@ -273,6 +290,24 @@ __device__ __forceinline__ void reset_bar(BarState* bar) {
 // Templated on the layout only so that each layout's unit has its own copy.
 template <class L>
 __global__ void k_reset_bar(BarState* bar) { reset_bar(bar); }
+
+// Before a batched launch (one block): resets the barrier and sets
+// bar->runs to whether any two consecutive batches together hold at most
+// run_cap ops, i.e. whether k_batches would form a run.
+template <class L>
+__global__ void k_reset_batches(BarState* bar, const uint64_t* offsets, uint64_t nbatches,
+                                uint64_t run_cap) {
+    reset_bar(bar);
+    __shared__ uint32_t any;
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    bool fits = false;
+    for (uint64_t i = threadIdx.x; i + 2 <= nbatches; i += blockDim.x)
+        fits |= offsets[i + 2] - offsets[i] <= run_cap;
+    if (fits) any = 1;
+    __syncthreads();
+    if (threadIdx.x == 0) st_coh(&bar->runs, any);
+}
 
 // ---------------------------------------------------------------------------
 // Served form: a device-resident request loop.

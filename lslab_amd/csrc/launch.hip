@@ -111,12 +111,21 @@ int Launch<L>::batches(const LaunchCtx& c, const OpsDev& o, const OutDev& out,
             constexpr bool BC = decltype(cc)::value;
             return dispatch_r(c.tun.reclaim, [&](auto rr) {
                 constexpr bool RC = decltype(rr)::value;
-                uint64_t grid = 0;
-                const int rc = persistent_grid(
-                    c, reinterpret_cast<const void*>(k_batches<L, G, BC, RC>), &grid);
+                uint64_t grid = 0, grid_r = 0;
+                int rc = persistent_grid(
+                    c, reinterpret_cast<const void*>(k_batches<L, G, BC, RC, false>), &grid);
+                if (rc == LSLAB_OK)
+                    rc = persistent_grid(
+                        c, reinterpret_cast<const void*>(k_batches<L, G, BC, RC, true>), &grid_r);
                 if (rc) return rc;
-                hipLaunchKernelGGL(k_reset_bar<L>, dim3(1), dim3(64), 0, s, bar);
-                hipLaunchKernelGGL((k_batches<L, G, BC, RC>), dim3((unsigned)grid),
+                // The reset kernel decides which of the two launches does the
+                // work (bar->runs); the other returns at once.
+                hipLaunchKernelGGL(k_reset_batches<L>, dim3(1), dim3(1024), 0, s, bar, offsets,
+                                   nbatches, c.run_cap);
+                hipLaunchKernelGGL((k_batches<L, G, BC, RC, false>), dim3((unsigned)grid),
+                                   dim3(PERSIST_THREADS), 0, s, c.dev, o, out, offsets, nbatches,
+                                   c.submax, c.run_cap, bar);
+                hipLaunchKernelGGL((k_batches<L, G, BC, RC, true>), dim3((unsigned)grid_r),
                                    dim3(PERSIST_THREADS), 0, s, c.dev, o, out, offsets, nbatches,
                                    c.submax, c.run_cap, bar);
                 return launched();
