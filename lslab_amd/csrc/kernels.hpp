@@ -189,7 +189,6 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_batches(TableDev t, OpsDev 
     const uint64_t gpb = blockDim.x / G;
     uint32_t* rc = rep_counter();
     uint32_t* cc = clr_counters();
-    RunOffs* ro = run_offs();
     RepEnt* rb = t.rep + block_rep_base_fixed(submax, ng, gpb, blockIdx.x);
     const uint64_t clr_base = run_clr_stride(run_cap, gridDim.x) * blockIdx.x;
     if (threadIdx.x == 0) {
@@ -215,31 +214,13 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_batches(TableDev t, OpsDev 
         run_phase_w<L, G, RC>(t, pt, o, plo, rb, *rc, rep_lds(), REP_LDS, pt.clr + clr_base,
                               cc[par ^ 1]);
     };
-    uint64_t b = 0;
-    while (b < nbatches) {
-        // The run [b, b + nb): one thread per boundary loads the window
-        // offsets[b .. b + RUN_MAX_BATCHES] into LDS, one thread scans it.
-        const uint64_t wn = min(RUNS ? (uint64_t)RUN_MAX_BATCHES : 1, nbatches - b);
-        if (threadIdx.x <= wn) ro->win[threadIdx.x] = offsets[b + threadIdx.x];
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            uint64_t e = 0;
-            while (e < wn && ro->win[e + 1] - ro->win[0] <= run_cap) ++e;
-            ro->nb = (int)e;
-        }
-        __syncthreads();
-        const uint64_t lo = ro->win[0];
-        const uint64_t nr = RUNS ? (uint64_t)ro->nb : 1;
-        if (nr <= 1) {  // batch b alone, in sub-batches of at most submax ops
-            const uint64_t hi = ro->win[1];
-            if constexpr (RUNS) {
-                if (pend) {
-                    flush_w();
-                    if (!grid_sync(bar, gridDim.x, t.spin_ticks, true, rc, &cc[par ^ 1]))
-                        return fail();
-                    pend = false;
-                }
-            }
+    if constexpr (!RUNS) {
+        // Every batch alone.  The end of the next batch is loaded while this
+        // batch runs, so a batch starts right after the previous barrier.
+        if (nbatches == 0) return;
+        uint64_t lo = offsets[0], hi = offsets[1];
+        for (uint64_t b = 0; b < nbatches; ++b) {
+            const uint64_t nxt = b + 2 <= nbatches ? offsets[b + 2] : hi;
             for (uint64_t slo = lo; slo < hi; slo += submax) {
                 const uint64_t m = min(submax, hi - slo);
                 phase_r_range<L, G, PERSIST_THREADS, BC>(t, o, out, slo, m, gid, ng, rb,
@@ -248,33 +229,89 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_batches(TableDev t, OpsDev 
                 phase_w_block<L, G, RC>(t, o, slo, rb, *rc, rep_lds(), REP_LDS);
                 if (!grid_sync(bar, gridDim.x, t.spin_ticks, true, rc)) return fail();
             }
-            __syncthreads();  // every wave has read the window before it is reloaded
-            b = b + 1;
+            lo = hi;
+            hi = nxt;
+        }
+        return;
+    }
+    // The window offsets[b .. b + RUN_MAX_BATCHES] of the run starting at
+    // batch b sits in LDS, ros[cur]; one thread per boundary loads it.  The
+    // next run's window is loaded before this run's first grid barrier and
+    // stored into ros[cur ^ 1] after it, so no run waits for its boundaries.
+    RunOffs* ros = run_offs();
+    int cur = 0;
+    if (nbatches == 0) return;
+    {
+        const uint64_t wn = min((uint64_t)RUN_MAX_BATCHES, nbatches);
+        if (threadIdx.x <= wn) ros[0].win[threadIdx.x] = offsets[threadIdx.x];
+        if (threadIdx.x == 0) ros[0].nb = (int)wn;
+        __syncthreads();
+    }
+    uint64_t b = 0;
+    while (b < nbatches) {
+        RunOffs* ro = &ros[cur];
+        // Batches in the run: the first e < wn whose end lies more than
+        // run_cap ops past the window's start (nb was preset to wn).
+        const uint64_t wn = min((uint64_t)RUN_MAX_BATCHES, nbatches - b);
+        if (threadIdx.x < wn && ro->win[threadIdx.x + 1] - ro->win[0] > run_cap)
+            atomicMin(&ro->nb, (int)threadIdx.x);
+        __syncthreads();
+        const uint64_t lo = ro->win[0];
+        const uint64_t nr = (uint64_t)ro->nb;
+        const uint64_t b2 = b + (nr <= 1 ? 1 : nr);  // the next run's first batch
+        const uint64_t wn2 = b2 < nbatches ? min((uint64_t)RUN_MAX_BATCHES, nbatches - b2) : 0;
+        const bool pfl = b2 < nbatches && threadIdx.x <= wn2;
+        uint64_t pf = 0;
+        auto pf_store = [&]() {
+            RunOffs* nx = &ros[cur ^ 1];
+            if (pfl) nx->win[threadIdx.x] = pf;
+            if (threadIdx.x == 0) nx->nb = (int)wn2;
+        };
+        if (nr <= 1) {  // batch b alone, in sub-batches of at most submax ops
+            const uint64_t hi = ro->win[1];
+            if (pend) {
+                flush_w();
+                if (!grid_sync(bar, gridDim.x, t.spin_ticks, true, rc, &cc[par ^ 1])) return fail();
+                pend = false;
+            }
+            if (pfl) pf = offsets[b2 + threadIdx.x];
+            if (hi == lo) pf_store();
+            for (uint64_t slo = lo; slo < hi; slo += submax) {
+                const uint64_t m = min(submax, hi - slo);
+                phase_r_range<L, G, PERSIST_THREADS, BC>(t, o, out, slo, m, gid, ng, rb,
+                                                         rep_lds(), REP_LDS);
+                if (!grid_sync(bar, gridDim.x, t.spin_ticks, false)) return fail();
+                if (slo == lo) pf_store();
+                phase_w_block<L, G, RC>(t, o, slo, rb, *rc, rep_lds(), REP_LDS);
+                if (!grid_sync(bar, gridDim.x, t.spin_ticks, true, rc)) return fail();
+            }
+            __syncthreads();  // the next window is in LDS
+            cur ^= 1;
+            b = b2;
             continue;
         }
-        if constexpr (RUNS) {
-            const uint64_t m = ro->win[nr] - lo;
-            if (threadIdx.x <= nr) ro->off[threadIdx.x] = ro->win[threadIdx.x] - lo;
-            __syncthreads();
-            const RunTabs rt = par ? t.run[1] : t.run[0];
-            if (pend) flush_w();  // the previous run's write-back, in the other table set
-            run_pass_a<L>(t, rt, o, out, lo, m, *ro, rt.clr + clr_base, &cc[par]);
-            // pass B reads the table (after that write-back) and the run tables
-            // with plain loads: drop stale lines
-            if (!grid_sync(bar, gridDim.x, t.spin_ticks, true, rc, &cc[par ^ 1])) return fail();
-            run_pass_b<L, G>(t, rt, o, out, lo, m, *ro, gid, ng, rb, rep_lds(), REP_LDS, rc);
-            if (!grid_sync(bar, gridDim.x, t.spin_ticks, false)) return fail();
-            pend = true;
-            plo = lo;
-            par ^= 1;
-            b += nr;
-        }
+        const uint64_t m = ro->win[nr] - lo;
+        if (threadIdx.x <= nr) ro->off[threadIdx.x] = ro->win[threadIdx.x] - lo;
+        __syncthreads();
+        const RunTabs rt = par ? t.run[1] : t.run[0];
+        if (pend) flush_w();  // the previous run's write-back, in the other table set
+        run_pass_a<L>(t, rt, o, out, lo, m, *ro, rt.clr + clr_base, &cc[par]);
+        if (pfl) pf = offsets[b2 + threadIdx.x];
+        // pass B reads the table (after that write-back) and the run tables
+        // with plain loads: drop stale lines
+        if (!grid_sync(bar, gridDim.x, t.spin_ticks, true, rc, &cc[par ^ 1])) return fail();
+        pf_store();
+        run_pass_b<L, G>(t, rt, o, out, lo, m, *ro, gid, ng, rb, rep_lds(), REP_LDS, rc);
+        if (!grid_sync(bar, gridDim.x, t.spin_ticks, false)) return fail();
+        pend = true;
+        plo = lo;
+        par ^= 1;
+        cur ^= 1;
+        b = b2;
     }
-    if constexpr (RUNS) {
-        if (pend) {
-            flush_w();
-            if (!grid_sync(bar, gridDim.x, t.spin_ticks, true, rc, &cc[par ^ 1])) return fail();
-        }
+    if (pend) {
+        flush_w();
+        if (!grid_sync(bar, gridDim.x, t.spin_ticks, true, rc, &cc[par ^ 1])) return fail();
     }
 }
 

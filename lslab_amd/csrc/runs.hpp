@@ -40,9 +40,10 @@ struct RunOffs {
     int nb;
 };
 
+// Two run windows: the current run's and the next run's (loaded ahead).
 __device__ __forceinline__ RunOffs* run_offs() {
-    __shared__ RunOffs ro;
-    return &ro;
+    __shared__ RunOffs ro[2];
+    return ro;
 }
 
 // The block's LDS counters of chain entries to clear, one per table set.
