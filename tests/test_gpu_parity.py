@@ -356,6 +356,56 @@ def test_exec_batches_uneven(lslab, oracle_mod, is_map, combine):
         np.testing.assert_array_equal(vv, rv)
 
 
+@pytest.mark.parametrize("lanes,combine", [(2, 1), (4, 2), (1, 1)])
+@pytest.mark.parametrize("is_map", [True, False])
+def test_exec_batches_all_alone(lslab, oracle_mod, is_map, lanes, combine):
+    """A stream in which no two consecutive batches fit one run (every
+    adjacent pair holds more than the 2^12-op run cap): the launcher's
+    reset kernel picks the kernel without the run path.  Batches larger
+    than a sub-batch, empty batches and Zipf conflicts; each batch checked
+    in order."""
+    say("exec_batches_all_alone", is_map, lanes, combine)
+    rng = np.random.default_rng(4097)
+    kb, vb = 8, 8
+    sub = 1 << 12
+    tbl = make(lslab, is_map, kb, vb, 6, 1 << 14, 1, lanes, subbatch=sub, combine=combine)
+    ref = Replay(is_map, kb, vb, 6, pool_slabs=1 << 14)
+    sizes = np.array([5000, 0, 9000, sub + 1, 0, 4500, 12000, sub + 1, 6000], np.int64)
+    assert all(sizes[i] + sizes[i + 1] > sub for i in range(len(sizes) - 1))
+    n = int(sizes.sum())
+    space = 8000
+    p = 1.0 / np.arange(1, space + 1) ** 0.99
+    p /= p.sum()
+    if is_map:
+        ranks = rng.choice(space, size=n, p=p)
+    else:
+        ranks = np.concatenate([rng.permutation(30000)[:s] for s in sizes])
+    keys = ranks.astype(np.uint64) * np.uint64(0x9E3779B97F4A7C15)
+    ops = rng.choice([0, 1, 2], size=n, p=[0.6, 0.3, 0.1]).astype(np.uint8)
+    vals = rng.integers(1, 2**63, size=n, dtype=np.uint64)
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    d_offs = torch.from_numpy(offs).cuda()
+    d_ops = torch.from_numpy(ops).cuda()
+    if is_map:
+        f, v = tbl.exec_batches(d_offs, d_ops, to_dev(keys, kb), to_dev(vals, vb))
+        f, v = f.cpu().numpy(), to_host(v.contiguous(), vb)
+    else:
+        f = tbl.exec_batches(d_offs, d_ops, to_dev(keys, kb)).cpu().numpy()
+        v = np.zeros(n, np.uint64)
+    assert tbl.status() == 0
+    for b in range(len(sizes)):
+        lo, hi = offs[b], offs[b + 1]
+        if hi == lo:
+            continue
+        bad, first = ref.check_batch(ops[lo:hi], keys[lo:hi], vals[lo:hi], f[lo:hi], v[lo:hi])
+        assert bad == 0, (b, first)
+    if is_map:
+        k, vv = dump_sorted(tbl, is_map, kb, vb)
+        rk, rv = oracle_dump_sorted(ref)
+        np.testing.assert_array_equal(k, rk)
+        np.testing.assert_array_equal(vv, rv)
+
+
 @pytest.mark.parametrize("lanes,max_blocks", [(8, 0), (8, 128), (1, 0)])
 def test_exec_batches_many_sizes(lslab, oracle_mod, lanes, max_blocks):
     """Thousands of uneven batches in one persistent launch: each block's
