@@ -212,7 +212,7 @@ int lslab_exec_batches(lslab_table* t, size_t nbatches, const uint64_t* offsets,
  * Same arguments, kernels and results as lslab_exec_batches, but the
  * batched kernel's scratch (grid-barrier state, combining table and
  * final-candidate lists) lives in one static device array of the library
- * (21 MiB per GPU, held until the process ends) instead of the table's.
+ * (about 25 MiB per GPU, held until the process ends) instead of the table's.
  * Limits: the table's max_subbatch must be at most
  * LSLAB_NOALLOC_MAX_SUBBATCH ops and its persistent grid (CUs, or max_blocks
  * when set) at most LSLAB_NOALLOC_MAX_BLOCKS blocks; otherwise the call

@@ -693,7 +693,7 @@ struct LaunchCtx {
 };
 
 // Ops a run of the batched kernel may hold: the sub-batch size, at most 2^20
-// (the run tables take 96 B per op of this capacity).
+// (the run tables take about 130 B per op of this capacity per set).
 constexpr uint64_t RUN_CAP_MAX = 1ull << 20;
 
 // Host launchers of one layout; each returns an LSLAB_* status.

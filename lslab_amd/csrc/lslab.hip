@@ -70,7 +70,7 @@ constexpr uint64_t NA_SET_BYTES = na_align(NA_RK_LEN * sizeof(RunKey)) +
 constexpr uint64_t NA_SET_OFF = na_align(NA_REP_OFF + NA_REP_LEN * sizeof(RepEnt));
 constexpr uint64_t NA_PRED_OFF = NA_SET_OFF + 2 * NA_SET_BYTES;
 constexpr uint64_t NA_BYTES = na_align(NA_PRED_OFF + NA_SUB * sizeof(uint32_t));
-static_assert(NA_BYTES <= 24ull << 20, "static scratch stays under 24 MiB");
+static_assert(NA_BYTES <= 32ull << 20, "static scratch stays under 32 MiB");
 }  // namespace
 
 static __device__ __align__(256) unsigned char scratch[NA_BYTES];
@@ -107,7 +107,7 @@ __global__ void k_noalloc_clear(uint64_t* gt_key, uint32_t* gt_head, RunKey* rk,
             gt_key[i] = GT_EMPTY;
             gt_head[i] = 0;
         }
-        if (i < NA_RK_LEN) rk[i] = RunKey{GT_EMPTY, 0};
+        if (i < NA_RK_LEN) rk[i] = RunKey{GT_EMPTY, {0, 0}, 0};
         rc[i] = RunChain{GT_EMPTY, 0, 0};
     }
 }
@@ -257,7 +257,7 @@ static uint64_t table_device_bytes(const lslab_table* t) {
 __global__ void k_init_run(RunKey* rk, uint64_t nk, RunChain* rc, uint64_t nc) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nk + nc;
          i += (uint64_t)gridDim.x * blockDim.x) {
-        if (i < nk) rk[i] = RunKey{GT_EMPTY, 0};
+        if (i < nk) rk[i] = RunKey{GT_EMPTY, {0, 0}, 0};
         else rc[i - nk] = RunChain{GT_EMPTY, 0, 0};
     }
 }

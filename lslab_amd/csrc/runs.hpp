@@ -28,7 +28,7 @@
 
 namespace lslab_dev {
 
-constexpr int RUN_MAX_BATCHES = 64;    // one bit per batch in a key's bitmap
+constexpr int RUN_MAX_BATCHES = 128;   // one bit per batch in a key's bitmap
 constexpr uint32_t RUN_CF = 0x80000000u;  // run_pred: first mutation of its chain
 constexpr uint32_t NOENT = 0xffffffffu;
 
@@ -113,6 +113,29 @@ __device__ __forceinline__ uint32_t rc_tail(const TableDev& t, const RunTabs& rt
     return 0;
 }
 
+// A key's 128-bit bitmap of the run's batches that mutate it.
+struct BatchMask {
+    uint64_t lo, hi;
+};
+
+// The bits of m below bit r.
+__device__ __forceinline__ BatchMask bm_below(BatchMask m, int r) {
+    if (r < 64) return BatchMask{m.lo & ((1ull << r) - 1ull), 0};
+    return BatchMask{m.lo, m.hi & ((1ull << (r - 64)) - 1ull)};
+}
+
+// Highest set bit of m, or -1.
+__device__ __forceinline__ int bm_top(BatchMask m) {
+    if (m.hi) return 127 - __builtin_clzll(m.hi);
+    return m.lo ? 63 - __builtin_clzll(m.lo) : -1;
+}
+
+// Whether no bit of m above bit r is set.
+__device__ __forceinline__ bool bm_none_above(BatchMask m, int r) {
+    if (r >= 64) return r == 127 || (m.hi >> (r - 63)) == 0;
+    return m.hi == 0 && (r == 63 || (m.lo >> (r + 1)) == 0);
+}
+
 // Claims or finds key's entry in the key table (the all-ones key: the last).
 __device__ __forceinline__ uint32_t rk_claim(const TableDev& t, const RunTabs& rt, uint64_t key) {
     if (key == GT_EMPTY) return t.rk_mask + 1u;
@@ -129,33 +152,43 @@ __device__ __forceinline__ uint32_t rk_claim(const TableDev& t, const RunTabs& r
 }
 
 // Key's entry and batch bitmap in the key table, plain loads (pass B).  The
-// first slot's 16-B entry was loaded ahead (w0); NOENT when the run does not
-// mutate the key.
+// first slot's entry (key and low mask word in w0, high mask word in w1) was
+// loaded ahead; NOENT when the run does not mutate the key.
 __device__ __forceinline__ uint32_t rk_lookup(const TableDev& t, const RunTabs& rt, uint64_t key, uint4 w0,
-                                              uint64_t& mask) {
-    mask = 0;
+                                              uint64_t w1, BatchMask& mask) {
+    mask = BatchMask{0, 0};
     if (key == GT_EMPTY) {
-        mask = rt.rk[t.rk_mask + 1u].mask;
-        return mask ? t.rk_mask + 1u : NOENT;
+        const RunKey& e = rt.rk[t.rk_mask + 1u];
+        mask = BatchMask{e.mask[0], e.mask[1]};
+        return (mask.lo | mask.hi) ? t.rk_mask + 1u : NOENT;
     }
     uint32_t h = (uint32_t)mix64(key) & t.rk_mask;
     uint4 w = w0;
+    uint64_t wh = w1;
     for (uint32_t p = 0; p <= t.rk_mask; ++p) {
         const uint64_t k = ((uint64_t)w.y << 32) | w.x;
         if (k == key) {
-            mask = ((uint64_t)w.w << 32) | w.z;
+            mask = BatchMask{((uint64_t)w.w << 32) | w.z, wh};
             return h;
         }
         if (k == GT_EMPTY) return NOENT;
         h = (h + 1u) & t.rk_mask;
         w = *reinterpret_cast<const uint4*>(&rt.rk[h]);
+        wh = rt.rk[h].mask[1];
     }
     return NOENT;
 }
 
-__device__ __forceinline__ uint4 rk_first(const TableDev& t, const RunTabs& rt, uint64_t key) {
-    if (key == GT_EMPTY) return make_uint4(0xffffffffu, 0xffffffffu, 0u, 0u);
-    return *reinterpret_cast<const uint4*>(&rt.rk[(uint32_t)mix64(key) & t.rk_mask]);
+__device__ __forceinline__ void rk_first(const TableDev& t, const RunTabs& rt, uint64_t key,
+                                         uint4& w0, uint64_t& w1) {
+    if (key == GT_EMPTY) {
+        w0 = make_uint4(0xffffffffu, 0xffffffffu, 0u, 0u);
+        w1 = 0;
+        return;
+    }
+    const RunKey* e = &rt.rk[(uint32_t)mix64(key) & t.rk_mask];
+    w0 = *reinterpret_cast<const uint4*>(e);
+    w1 = e->mask[1];
 }
 
 __device__ __forceinline__ void corrupt(const TableDev& t) {
@@ -252,8 +285,8 @@ __device__ __forceinline__ void run_a_round(const TableDev& t, const RunTabs& rt
             if (won) {
                 const uint32_t gk = rk_claim(t, rt, kk);
                 if (gk != NOENT)
-                    __hip_atomic_fetch_or(&rt.rk[gk].mask, 1ull << r, __ATOMIC_RELAXED,
-                                          __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_fetch_or(&rt.rk[gk].mask[r >> 6], 1ull << (r & 63),
+                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 else
                     corrupt(t);
             }
@@ -368,23 +401,26 @@ __device__ __forceinline__ void run_b_op(const TableDev& t, const RunTabs& rt, c
     K blk[S];
     probe_issue<L, G, false>(t, b, blk);
     uint4 w0 = make_uint4(0, 0, 0, 0);
-    if (gl == 0) w0 = rk_first(t, rt, kk);
+    uint64_t w1 = 0;
+    if (gl == 0) rk_first(t, rt, kk, w0, w1);
     const int r = run_batch_of(ro, li);
-    uint64_t mask = 0;
+    BatchMask mask{0, 0};
     uint32_t gk = NOENT;
-    if (gl == 0) gk = rk_lookup(t, rt, kk, w0, mask);
-    mask = grp_bcast(mask, 0, G);
+    if (gl == 0) gk = rk_lookup(t, rt, kk, w0, w1, mask);
+    mask.lo = grp_bcast(mask.lo, 0, G);
+    mask.hi = grp_bcast(mask.hi, 0, G);
     gk = grp_bcast(gk, 0, G);
-    const uint64_t earlier = mask & ((1ull << r) - 1ull);
-    const bool last = op != OP_GET && (r == RUN_MAX_BATCHES - 1 || (mask >> (r + 1)) == 0);
+    const int r2 = bm_top(bm_below(mask, r));  // nearest earlier batch that mutates the key
+    const bool earlier = r2 >= 0;
+    const bool last = op != OP_GET && bm_none_above(mask, r);
     V cur = V(0);
     uint64_t s = b;
     int slot = -1;
-    if (earlier == 0 || last) {
+    if (!earlier || last) {
         uint32_t valid;
         walk_on<L, G, false>(t, probe_compute<L, G, false>(t, b, key, blk), s, slot, valid, key);
         if constexpr (L::IS_MAP) {
-            if (slot >= 0 && earlier == 0 && gl == 0) cur = *val_slot<L>(t, s, slot);
+            if (slot >= 0 && !earlier && gl == 0) cur = *val_slot<L>(t, s, slot);
         }
     }
     if (gl != 0) return;
@@ -392,7 +428,6 @@ __device__ __forceinline__ void run_b_op(const TableDev& t, const RunTabs& rt, c
     bool found = in_table;
     uint32_t p = 0;  // predecessor: the tail of the nearest earlier batch's chain
     if (earlier) {
-        const int r2 = 63 - __builtin_clzll(earlier);
         uint64_t base;
         uint32_t cmask;
         run_region(ro, r2, base, cmask);
@@ -482,7 +517,8 @@ __device__ __forceinline__ void run_phase_w(const TableDev& t, const RunTabs& rt
         const V val = (L::IS_MAP && o.vals) ? ld_stride<V>(o.vals, o.vstride, i) : V(0);
         if (gl == 0) {
             st_coh(&rt.rk[e.g].key, GT_EMPTY);
-            st_coh(&rt.rk[e.g].mask, (uint64_t)0);
+            st_coh(&rt.rk[e.g].mask[0], (uint64_t)0);
+            st_coh(&rt.rk[e.g].mask[1], (uint64_t)0);
         }
         w_apply<L, G, RC>(t, op, key, val, e.loc);
     }

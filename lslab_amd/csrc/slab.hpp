@@ -76,11 +76,12 @@ struct RepEnt {
 // Scratch of a run of batches in the persistent batched kernel.  The key
 // table holds, per key the run mutates, the bitmap of the run's batches that
 // mutate it; the chain table holds, per (key, batch), the last mutation of
-// that batch's exchange chain.  Key and payload share one 16-B entry, so one
-// load reads both.
-struct RunKey {
-    uint64_t key;   // GT_EMPTY = free
-    uint64_t mask;  // bit r: batch r of the run mutates the key
+// that batch's exchange chain.  Key and payload share one entry (32 B
+// aligned, so one cache sector holds it).
+struct alignas(32) RunKey {
+    uint64_t key;      // GT_EMPTY = free
+    uint64_t mask[2];  // bit r (word r / 64): batch r of the run mutates the key
+    uint64_t pad;
 };
 struct RunChain {
     uint64_t key;   // GT_EMPTY = free

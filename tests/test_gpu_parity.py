@@ -707,6 +707,55 @@ def test_exec_batches_run_boundaries(lslab, oracle_mod, is_map):
 
 
 @pytest.mark.parametrize("combine", [1, 2])
+@pytest.mark.parametrize("is_map", [True, False])
+def test_exec_batches_long_runs(lslab, oracle_mod, is_map, combine):
+    """Runs of the full 128 batches (small batches, far below the 2^16-op run
+    cap) over a small, Zipf-hot key space, so that one key is mutated in
+    batches on both sides of batch 64 of a run: the high word of the key
+    table's batch bitmap carries the nearest earlier batch and the key's
+    last batch.  Each batch checked in order, then the final contents."""
+    say("exec_batches_long_runs", is_map, combine)
+    rng = np.random.default_rng(128 + combine)
+    kb, vb = 8, 8
+    tbl = make(lslab, is_map, kb, vb, 6, 1 << 14, 1, 2, subbatch=1 << 16, combine=combine)
+    ref = Replay(is_map, kb, vb, 6, pool_slabs=1 << 14)
+    sizes = rng.integers(20, 200, size=400).astype(np.int64)
+    sizes[[5, 70, 130, 300]] = 0  # empty batches inside runs
+    n = int(sizes.sum())
+    space = 300
+    p = 1.0 / np.arange(1, space + 1) ** 1.1
+    p /= p.sum()
+    if is_map:
+        ranks = rng.choice(space, size=n, p=p)
+    else:
+        ranks = np.concatenate([rng.permutation(space)[:s] for s in sizes])
+    keys = ranks.astype(np.uint64) * np.uint64(0x9E3779B97F4A7C15)
+    ops = rng.choice([0, 1, 2], size=n, p=[0.5, 0.35, 0.15]).astype(np.uint8)
+    vals = rng.integers(1, 2**63, size=n, dtype=np.uint64)
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    d_offs = torch.from_numpy(offs).cuda()
+    d_ops = torch.from_numpy(ops).cuda()
+    if is_map:
+        f, v = tbl.exec_batches(d_offs, d_ops, to_dev(keys, kb), to_dev(vals, vb))
+        f, v = f.cpu().numpy(), to_host(v.contiguous(), vb)
+    else:
+        f = tbl.exec_batches(d_offs, d_ops, to_dev(keys, kb)).cpu().numpy()
+        v = np.zeros(n, np.uint64)
+    assert tbl.status() == 0
+    for b in range(len(sizes)):
+        lo, hi = offs[b], offs[b + 1]
+        if hi == lo:
+            continue
+        bad, first = ref.check_batch(ops[lo:hi], keys[lo:hi], vals[lo:hi], f[lo:hi], v[lo:hi])
+        assert bad == 0, (b, first)
+    if is_map:
+        k, vv = dump_sorted(tbl, is_map, kb, vb)
+        rk, rv = oracle_dump_sorted(ref)
+        np.testing.assert_array_equal(k, rk)
+        np.testing.assert_array_equal(vv, rv)
+
+
+@pytest.mark.parametrize("combine", [1, 2])
 def test_exec_batches_reclaim_runs(lslab, oracle_mod, combine):
     """The batched kernel on a reclaim table: many small batches (runs sharing
     one phase W) that insert and remove heavily in few buckets, so pool slabs
