@@ -704,9 +704,11 @@ def main():
     if st:
         raise RuntimeError(f"plain status {st}")
 
+    # Batches of max_subbatch ops never share a run: the call skips the
+    # device-side check and launches the all-alone kernel only.
     m.set_tuning(lslab_amd.tuning(lanes_per_op=th["lanes_per_op"], engine=lslab_amd.ENGINE_TWO_PHASE,
                                   max_blocks=th["max_blocks"], max_subbatch=BATCH,
-                                  combine=th["combine"]))
+                                  combine=th["combine"], runs=lslab_amd.RUNS_OFF))
     time_fused(m, ops, keys, vals, offs[: W + 1], W, torch)
     offs_k = offs[W:] - W * BATCH
     wall_f, kern_f = time_fused(m, ops[W * BATCH:], keys[W * BATCH:], vals[W * BATCH:], offs_k, K,

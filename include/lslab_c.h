@@ -95,6 +95,9 @@ extern "C" {
  * slabs has none. */
 #define LSLAB_SLOT_HEADS_ON 1
 #define LSLAB_SLOT_HEADS_OFF 2
+/* Runs of batches in lslab_exec_batches (lslab_tuning.runs). */
+#define LSLAB_RUNS_ON 1
+#define LSLAB_RUNS_OFF 2
 
 typedef struct lslab_table lslab_table;
 
@@ -114,6 +117,14 @@ typedef struct {
                               for that long ends its kernel with LSLAB_ERR_TIMEOUT.  The
                               served loop's wait for an arrival is not bounded by it. */
     int slot_heads;        /* LSLAB_SLOT_HEADS_* (0 = AUTO: ON while engine is TWO_PHASE) */
+    int runs;              /* LSLAB_RUNS_* for lslab_exec_batches.  0 = AUTO: the call checks on
+                              the device whether any two consecutive batches fit one run
+                              (max_subbatch ops, at most 2^20) and launches the run kernel and
+                              the all-alone kernel, of which the one that does not apply
+                              returns at once.  ON: the run kernel only.  OFF: the all-alone
+                              kernel only (every batch runs by itself; for streams whose
+                              batches never fit a run, one launch less).  Every setting gives
+                              the same results. */
 } lslab_tuning;
 
 typedef struct {

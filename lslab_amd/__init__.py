@@ -7,11 +7,12 @@ Python package binds the same C ABI for tests and benchmarks.
 from . import _lib  # noqa: F401
 from ._lib import (COMBINE_BLOCK, COMBINE_WAVE, ENGINE_PLAIN, ENGINE_TWO_PHASE,  # noqa: F401
                    HASH_IDENTITY, HASH_MIX, OP_GET, OP_PUT, OP_REMOVE, RECLAIM_OFF,
-                   RECLAIM_ON, SLOT_HEADS_OFF, SLOT_HEADS_ON, LslabError)
+                   RECLAIM_ON, RUNS_OFF, RUNS_ON, SLOT_HEADS_OFF, SLOT_HEADS_ON, LslabError)
 
 __all__ = ["Map", "Set", "Kvcg", "Wide", "tuning", "slab_bytes", "workload", "ENGINE_PLAIN", "ENGINE_TWO_PHASE",
            "COMBINE_WAVE", "COMBINE_BLOCK", "HASH_IDENTITY", "HASH_MIX", "RECLAIM_OFF",
-           "RECLAIM_ON", "SLOT_HEADS_ON", "SLOT_HEADS_OFF", "OP_GET", "OP_PUT", "OP_REMOVE",
+           "RECLAIM_ON", "SLOT_HEADS_ON", "SLOT_HEADS_OFF", "RUNS_ON", "RUNS_OFF", "OP_GET",
+           "OP_PUT", "OP_REMOVE",
            "LslabError"]
 
 

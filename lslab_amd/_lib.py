@@ -33,6 +33,8 @@ RECLAIM_OFF = 1
 RECLAIM_ON = 2
 SLOT_HEADS_ON = 1
 SLOT_HEADS_OFF = 2
+RUNS_ON = 1
+RUNS_OFF = 2
 
 CHAIN_BINS = 16
 
@@ -69,6 +71,7 @@ class Tuning(ctypes.Structure):
         ("reclaim", ctypes.c_int),
         ("spin_ms", ctypes.c_int),
         ("slot_heads", ctypes.c_int),
+        ("runs", ctypes.c_int),
     ]
 
 

@@ -329,20 +329,23 @@ template <class L>
 __global__ void k_reset_bar(BarState* bar) { reset_bar(bar); }
 
 // Before a batched launch (one block): resets the barrier and sets
-// bar->runs to whether any two consecutive batches together hold at most
-// run_cap ops, i.e. whether k_batches would form a run.
+// bar->runs.  With mode LSLAB_RUNS_ON or _OFF that is 1 or 0; with 0 (auto)
+// it is whether any two consecutive batches together hold at most run_cap
+// ops, i.e. whether k_batches would form a run.
 template <class L>
 __global__ void k_reset_batches(BarState* bar, const uint64_t* offsets, uint64_t nbatches,
-                                uint64_t run_cap) {
+                                uint64_t run_cap, int mode) {
     reset_bar(bar);
     __shared__ uint32_t any;
-    if (threadIdx.x == 0) any = 0;
+    if (threadIdx.x == 0) any = mode == LSLAB_RUNS_ON ? 1u : 0u;
     __syncthreads();
-    bool fits = false;
-    for (uint64_t i = threadIdx.x; i + 2 <= nbatches; i += blockDim.x)
-        fits |= offsets[i + 2] - offsets[i] <= run_cap;
-    if (fits) any = 1;
-    __syncthreads();
+    if (mode == 0) {
+        bool fits = false;
+        for (uint64_t i = threadIdx.x; i + 2 <= nbatches; i += blockDim.x)
+            fits |= offsets[i + 2] - offsets[i] <= run_cap;
+        if (fits) any = 1;
+        __syncthreads();
+    }
     if (threadIdx.x == 0) st_coh(&bar->runs, any);
 }
 

@@ -1,6 +1,9 @@
 // lslab: host launchers of one table layout.  The build compiles this unit
 // once per layout with -DLSLAB_LAYOUT=<layout alias>, so that the six
 // layouts' kernels are separate translation units built in parallel.
+#include <mutex>
+#include <unordered_map>
+
 #include "kernels.hpp"
 
 #ifndef LSLAB_LAYOUT
@@ -54,12 +57,25 @@ uint64_t grid_for(const LaunchCtx& c, uint64_t n) {
 // (launch bounds 1024 cap them at 128 VGPRs; their ~106 SGPRs allow 7 waves
 // per SIMD), and the occupancy query must confirm at least one block per CU.
 // max_blocks (tuning) can shrink the grid for small batches.
+// The occupancy answer depends on the kernel only, so it is asked once per
+// kernel and process and kept.
 int persistent_grid(const LaunchCtx& c, const void* kernel, uint64_t* grid) {
+    static std::mutex mu;
+    static std::unordered_map<const void*, int> checked;
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, PERSIST_THREADS, 0) !=
-            hipSuccess ||
-        per_cu < 1)
-        return LSLAB_ERR_HIP;
+    {
+        std::lock_guard<std::mutex> g(mu);
+        auto it = checked.find(kernel);
+        if (it != checked.end()) per_cu = it->second;
+    }
+    if (per_cu == 0) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, PERSIST_THREADS, 0) !=
+            hipSuccess)
+            return LSLAB_ERR_HIP;
+        std::lock_guard<std::mutex> g(mu);
+        checked[kernel] = per_cu;
+    }
+    if (per_cu < 1) return LSLAB_ERR_HIP;
     uint64_t g = (uint64_t)c.num_cus;
     if (c.tun.max_blocks > 0) g = std::min<uint64_t>(g, (uint64_t)c.tun.max_blocks);
     *grid = g;
@@ -111,23 +127,30 @@ int Launch<L>::batches(const LaunchCtx& c, const OpsDev& o, const OutDev& out,
             constexpr bool BC = decltype(cc)::value;
             return dispatch_r(c.tun.reclaim, [&](auto rr) {
                 constexpr bool RC = decltype(rr)::value;
+                const int mode = c.tun.runs;
+                const bool alone = mode != LSLAB_RUNS_ON, runs = mode != LSLAB_RUNS_OFF;
                 uint64_t grid = 0, grid_r = 0;
-                int rc = persistent_grid(
-                    c, reinterpret_cast<const void*>(k_batches<L, G, BC, RC, false>), &grid);
-                if (rc == LSLAB_OK)
+                int rc = LSLAB_OK;
+                if (alone)
+                    rc = persistent_grid(
+                        c, reinterpret_cast<const void*>(k_batches<L, G, BC, RC, false>), &grid);
+                if (rc == LSLAB_OK && runs)
                     rc = persistent_grid(
                         c, reinterpret_cast<const void*>(k_batches<L, G, BC, RC, true>), &grid_r);
                 if (rc) return rc;
-                // The reset kernel decides which of the two launches does the
-                // work (bar->runs); the other returns at once.
+                // The reset kernel sets bar->runs (found out on the device in
+                // auto mode); of two launches the one that does not match it
+                // returns at once.
                 hipLaunchKernelGGL(k_reset_batches<L>, dim3(1), dim3(1024), 0, s, bar, offsets,
-                                   nbatches, c.run_cap);
-                hipLaunchKernelGGL((k_batches<L, G, BC, RC, false>), dim3((unsigned)grid),
-                                   dim3(PERSIST_THREADS), 0, s, c.dev, o, out, offsets, nbatches,
-                                   c.submax, c.run_cap, bar);
-                hipLaunchKernelGGL((k_batches<L, G, BC, RC, true>), dim3((unsigned)grid_r),
-                                   dim3(PERSIST_THREADS), 0, s, c.dev, o, out, offsets, nbatches,
-                                   c.submax, c.run_cap, bar);
+                                   nbatches, c.run_cap, mode);
+                if (alone)
+                    hipLaunchKernelGGL((k_batches<L, G, BC, RC, false>), dim3((unsigned)grid),
+                                       dim3(PERSIST_THREADS), 0, s, c.dev, o, out, offsets,
+                                       nbatches, c.submax, c.run_cap, bar);
+                if (runs)
+                    hipLaunchKernelGGL((k_batches<L, G, BC, RC, true>), dim3((unsigned)grid_r),
+                                       dim3(PERSIST_THREADS), 0, s, c.dev, o, out, offsets,
+                                       nbatches, c.submax, c.run_cap, bar);
                 return launched();
             });
         });

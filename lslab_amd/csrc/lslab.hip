@@ -160,6 +160,7 @@ static bool tuning_ok(const lslab_tuning& tu) {
     if (tu.slot_heads != 0 && tu.slot_heads != LSLAB_SLOT_HEADS_ON &&
         tu.slot_heads != LSLAB_SLOT_HEADS_OFF)
         return false;
+    if (tu.runs != 0 && tu.runs != LSLAB_RUNS_ON && tu.runs != LSLAB_RUNS_OFF) return false;
     return true;
 }
 

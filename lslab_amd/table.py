@@ -59,13 +59,14 @@ def slab_bytes(is_map, key_bytes, value_bytes, n_log2, pool_slabs):
 
 def tuning(lanes_per_op=4, engine=_lib.ENGINE_PLAIN, block_threads=256, max_blocks=0,
            max_subbatch=1 << 20, combine=_lib.COMBINE_BLOCK, hash=0, reclaim=0, spin_ms=0,
-           slot_heads=0):
+           slot_heads=0, runs=0):
     """Launch shapes and table policies (lslab_tuning).  hash and reclaim are
     fixed at creation; 0 in set_tuning keeps them.  spin_ms bounds every device
     spin (0 = 10 s).  slot_heads: SLOT_HEADS_ON / SLOT_HEADS_OFF, 0 = on while
-    engine is two-phase."""
+    engine is two-phase.  runs (exec_batches): RUNS_ON / RUNS_OFF, 0 = found out
+    per call on the device."""
     return _lib.Tuning(lanes_per_op, engine, block_threads, max_blocks, max_subbatch, combine,
-                       hash, reclaim, spin_ms, slot_heads)
+                       hash, reclaim, spin_ms, slot_heads, runs)
 
 
 class _Table:

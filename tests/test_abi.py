@@ -71,6 +71,7 @@ def test_cpp_headers_compile():
            ' static_assert(sizeof(lslab::key_value<unsigned,unsigned long long>)==16,"");'
            f' static_assert(sizeof(lslab_tuning)=={ctypes.sizeof(_lib.Tuning)},"");'
            f' static_assert(offsetof(lslab_tuning, slot_heads)=={_lib.Tuning.slot_heads.offset},"");'
+           f' static_assert(offsetof(lslab_tuning, runs)=={_lib.Tuning.runs.offset},"");'
            ' return 0; }\n')
     with tempfile.TemporaryDirectory() as d:
         f = os.path.join(d, "t.cpp")

@@ -706,6 +706,42 @@ def test_exec_batches_run_boundaries(lslab, oracle_mod, is_map):
         np.testing.assert_array_equal(vv, rv)
 
 
+@pytest.mark.parametrize("runs", [0, 1, 2])
+def test_exec_batches_runs_modes(lslab, oracle_mod, runs):
+    """lslab_tuning.runs: auto, run kernel only, all-alone kernel only.  The
+    same stream (small batches that form runs, batches larger than a
+    sub-batch, an empty batch) gives oracle-correct results in every mode."""
+    say("exec_batches_runs_modes", runs)
+    rng = np.random.default_rng(3003)
+    kb = vb = 8
+    tbl = lslab.Map(kb, vb, 6, 1 << 14, lslab.tuning(lanes_per_op=2, engine=1,
+                                                     max_subbatch=1 << 12, combine=1, runs=runs))
+    ref = Replay(True, kb, vb, 6, pool_slabs=1 << 14)
+    sizes = np.array([30, 500, 7, 0, 9000, 64, 1, 4096, 200, 5000, 40], np.int64)
+    n = int(sizes.sum())
+    space = 2000
+    p = 1.0 / np.arange(1, space + 1) ** 0.99
+    p /= p.sum()
+    keys = rng.choice(space, size=n, p=p).astype(np.uint64) * np.uint64(0x9E3779B97F4A7C15)
+    ops = rng.choice([0, 1, 2], size=n, p=[0.6, 0.3, 0.1]).astype(np.uint8)
+    vals = rng.integers(1, 2**63, size=n, dtype=np.uint64)
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    f, v = tbl.exec_batches(torch.from_numpy(offs).cuda(), torch.from_numpy(ops).cuda(),
+                            to_dev(keys, kb), to_dev(vals, vb))
+    f, v = f.cpu().numpy(), to_host(v.contiguous(), vb)
+    assert tbl.status() == 0
+    for b in range(len(sizes)):
+        lo, hi = offs[b], offs[b + 1]
+        if hi == lo:
+            continue
+        bad, first = ref.check_batch(ops[lo:hi], keys[lo:hi], vals[lo:hi], f[lo:hi], v[lo:hi])
+        assert bad == 0, (b, first)
+    k, vv = dump_sorted(tbl, True, kb, vb)
+    rk, rv = oracle_dump_sorted(ref)
+    np.testing.assert_array_equal(k, rk)
+    np.testing.assert_array_equal(vv, rv)
+
+
 @pytest.mark.parametrize("combine", [1, 2])
 @pytest.mark.parametrize("is_map", [True, False])
 def test_exec_batches_long_runs(lslab, oracle_mod, is_map, combine):
