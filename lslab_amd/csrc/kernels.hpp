@@ -379,6 +379,7 @@ struct ServeState {
     uint64_t t0;
     uint64_t next_req;
     uint64_t r_lo, r_hi;
+    uint64_t op_lo, op_hi;  // the tick's op range: off[r_lo], off[r_hi]
     uint64_t ticks;
     uint64_t rejected;
     uint64_t served_ops;
@@ -423,16 +424,38 @@ __device__ __forceinline__ uint64_t block_scan(uint64_t v, uint64_t* total) {
 // fit the op cap, the request cap and the pool headroom.  These are the
 // rules of a request-by-request admission in arrival order, with the clock
 // read once per chunk.
+// Block 0's scheduler state for the whole launch (its LDS): the next request
+// to consider, and an upper bound of the slabs in use once every admitted
+// tick has finished (the last exact count plus the reservations admitted
+// since).
+struct SchedState {
+    uint64_t r;
+    uint64_t bound;
+};
+
+__device__ __forceinline__ SchedState* sched_state() {
+    __shared__ SchedState ss;
+    return &ss;
+}
+
+// exact: no admitted tick is still writing back, so the pool's counters are
+// final and are read; otherwise (block 0 schedules the next tick while the
+// other blocks finish the current tick's phase W) the headroom check uses
+// the bound, and a request the bound refuses is not admitted yet: it waits
+// for a later schedule, at the latest the exact one that an empty tick
+// leads to.  A request whose reservation exceeds the pool's exact headroom
+// can never fit (the pool only shrinks) and is rejected.
 __device__ inline void serve_schedule(const TableDev& t, const ServeDev& sv, ServeState* ss,
-                                      BarState* bar) {
-    __shared__ uint64_t st_r, st_lo, st_n, st_ops, st_res, st_used, st_el;
+                                      BarState* bar, bool exact) {
+    __shared__ uint64_t st_r, st_lo, st_n, st_ops, st_res, st_used, st_el, st_off;
     __shared__ uint64_t st_pos[PERSIST_THREADS / 64];
     __shared__ uint64_t st_ps, st_pn;
     __shared__ int st_go;
+    SchedState* S = sched_state();
     if (threadIdx.x == 0) {
-        uint64_t r = ss->next_req;
-        const uint64_t used = ld_agent(t.alloc_ctr) - ld_agent(t.free_count);
-        uint64_t lo = r, n = 0, ops = 0, reserved = 0;
+        uint64_t r = S->r;
+        const uint64_t used = exact ? ld_agent(t.alloc_ctr) - ld_agent(t.free_count) : S->bound;
+        uint64_t lo = r, n = 0, ops = 0, reserved = 0, off0 = 0;
         while (r < sv.nreq) {
             if (sv.arrival[r] > now_ticks() - ss->t0) {
                 // Idle: wait for the next arrival, however far away, with a
@@ -450,8 +473,10 @@ __device__ inline void serve_schedule(const TableDev& t, const ServeDev& sv, Ser
                     __builtin_amdgcn_s_sleep(8);
                 }
             }
-            const uint64_t sz = sv.off[r + 1] - sv.off[r];
+            off0 = sv.off[r];
+            const uint64_t sz = sv.off[r + 1] - off0;
             const uint64_t need = sv.reserve ? sv.reserve[r] : sz;
+            if (used + need > t.pool && !exact) break;  // wait for the exact count
             if (used + need > t.pool) {  // can never fit: the pool only shrinks
                 sv.tick_of[r] = SERVE_REJECTED;
                 sv.complete[r] = now_ticks() - ss->t0;
@@ -472,6 +497,7 @@ __device__ inline void serve_schedule(const TableDev& t, const ServeDev& sv, Ser
         st_ops = ops;
         st_res = reserved;
         st_used = used;
+        st_off = off0;
         st_go = n > 0 && r < sv.nreq;
     }
     __syncthreads();
@@ -521,12 +547,20 @@ __device__ inline void serve_schedule(const TableDev& t, const ServeDev& sv, Ser
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        // Read by every block after the barrier: agent-scope stores.
+        // Read by every block after the barrier: agent-scope stores.  The
+        // admitted requests are contiguous, so their ops are
+        // [off[r_lo], off[r_lo] + ops).  No admission with requests left is
+        // an empty tick; with none left the loop is done.
+        const uint64_t op_lo = st_n ? st_off : 0;
         st_coh(&ss->r_lo, st_lo);
         st_coh(&ss->r_hi, st_r);
+        st_coh(&ss->op_lo, op_lo);
+        st_coh(&ss->op_hi, op_lo + st_ops);
         ss->next_req = st_r;
         ss->served_ops += st_ops;
-        if (st_n == 0) st_coh(&ss->done, 1u);
+        if (st_n == 0 && st_r >= sv.nreq) st_coh(&ss->done, 1u);
+        S->r = st_r;
+        S->bound = st_used + st_res;  // the admitted tick allocates at most its reservations
     }
 }
 
@@ -543,36 +577,50 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_serve(TableDev t, OpsDev o,
         comb_init<PERSIST_THREADS>();
     else
         __syncthreads();
-    if (blockIdx.x == 0 && threadIdx.x == 0) ss->t0 = now_ticks();
-    for (uint64_t tick = 0;; ++tick) {
-        if (blockIdx.x == 0) serve_schedule(t, sv, ss, bar);  // block-uniform
-        if (!grid_sync(bar, gridDim.x, t.spin_ticks, false)) goto fail;
-        {
-            if (ld_agent(&ss->done)) break;
-            const uint64_t r_lo = ld_coh(&ss->r_lo), r_hi = ld_coh(&ss->r_hi);
-            const uint64_t lo = sv.off[r_lo], hi = sv.off[r_hi];
-            for (uint64_t slo = lo; slo < hi; slo += submax) {
-                const uint64_t m = min(submax, hi - slo);
-                phase_r_range<L, G, PERSIST_THREADS, BC>(t, o, out, slo, m, gid, ng, rb, rep_lds(),
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ss->t0 = now_ticks();
+        sched_state()->r = 0;
+        sched_state()->bound = 0;
+    }
+    // Block 0 admits the next tick at the top of the loop.  After a tick it
+    // does so as soon as its own phase W is done, while the other blocks
+    // finish theirs: the grid barrier that ends the write-back also
+    // publishes the next tick (two grid barriers per tick).
+    bool running = false;  // a tick's phase W is still running on other blocks
+    uint64_t ntick = 0;    // ticks with ops
+    for (;;) {
+        if (blockIdx.x == 0) serve_schedule(t, sv, ss, bar, !running);  // block-uniform
+        if (!grid_sync(bar, gridDim.x, t.spin_ticks, running, rc)) goto fail;
+        running = false;
+        if (ld_agent(&ss->done)) break;
+        const uint64_t r_lo = ld_coh(&ss->r_lo), r_hi = ld_coh(&ss->r_hi);
+        const uint64_t lo = ld_coh(&ss->op_lo), hi = ld_coh(&ss->op_hi);
+        if (lo == hi) continue;  // empty tick: schedule again, exactly
+        for (uint64_t slo = lo;;) {
+            const uint64_t m = min(submax, hi - slo);
+            phase_r_range<L, G, PERSIST_THREADS, BC>(t, o, out, slo, m, gid, ng, rb, rep_lds(),
                                                      REP_LDS);
-                if (!grid_sync(bar, gridDim.x, t.spin_ticks, false)) goto fail;
-                // Every result of the tick is written in phase R: its requests
-                // complete here.  Phase W only brings the table up to date for
-                // the next tick.
-                if (slo + m >= hi && blockIdx.x == 0) {
-                    if (threadIdx.x == 0) stamp_s = now_ticks() - ss->t0;
-                    __syncthreads();
-                    for (uint64_t r = r_lo + threadIdx.x; r < r_hi; r += blockDim.x) {
-                        sv.complete[r] = stamp_s;
-                        sv.tick_of[r] = (uint32_t)tick;
-                    }
-                    __syncthreads();
-                    if (threadIdx.x == 0) ss->ticks = tick + 1;
+            if (!grid_sync(bar, gridDim.x, t.spin_ticks, false)) goto fail;
+            // Every result of the tick is written in phase R: its requests
+            // complete here.  Phase W only brings the table up to date for
+            // the next tick.
+            if (slo + m >= hi && blockIdx.x == 0) {
+                if (threadIdx.x == 0) stamp_s = now_ticks() - ss->t0;
+                __syncthreads();
+                for (uint64_t r = r_lo + threadIdx.x; r < r_hi; r += blockDim.x) {
+                    sv.complete[r] = stamp_s;
+                    sv.tick_of[r] = (uint32_t)ntick;
                 }
-                phase_w_block<L, G, RC>(t, o, slo, rb, *rc, rep_lds(), REP_LDS);
-                if (!grid_sync(bar, gridDim.x, t.spin_ticks, true, rc)) goto fail;
+                __syncthreads();
+                if (threadIdx.x == 0) ss->ticks = ntick + 1;
             }
+            phase_w_block<L, G, RC>(t, o, slo, rb, *rc, rep_lds(), REP_LDS);
+            slo += m;
+            if (slo >= hi) break;
+            if (!grid_sync(bar, gridDim.x, t.spin_ticks, true, rc)) goto fail;
         }
+        ++ntick;
+        running = true;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) ss->end = now_ticks() - ss->t0;
     return;
