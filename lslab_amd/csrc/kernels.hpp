@@ -578,7 +578,7 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_serve(TableDev t, OpsDev o,
     else
         __syncthreads();
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        ss->t0 = now_ticks();
+        st_coh(&ss->t0, now_ticks());  // read by the stamping block
         sched_state()->r = 0;
         sched_state()->bound = 0;
     }
@@ -604,8 +604,9 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_serve(TableDev t, OpsDev o,
             // Every result of the tick is written in phase R: its requests
             // complete here.  Phase W only brings the table up to date for
             // the next tick.
-            if (slo + m >= hi && blockIdx.x == 0) {
-                if (threadIdx.x == 0) stamp_s = now_ticks() - ss->t0;
+            // The last block stamps them (block 0 has the scheduler's work).
+            if (slo + m >= hi && blockIdx.x == gridDim.x - 1) {
+                if (threadIdx.x == 0) stamp_s = now_ticks() - ld_coh(&ss->t0);
                 __syncthreads();
                 for (uint64_t r = r_lo + threadIdx.x; r < r_hi; r += blockDim.x) {
                     sv.complete[r] = stamp_s;
