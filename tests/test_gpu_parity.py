@@ -791,6 +791,57 @@ def test_exec_batches_long_runs(lslab, oracle_mod, is_map, combine):
         np.testing.assert_array_equal(vv, rv)
 
 
+@pytest.mark.parametrize("lanes", [1, 4])
+@pytest.mark.parametrize("is_map", [True, False])
+def test_exec_batches_crowded_key_table(lslab, oracle_mod, is_map, lanes):
+    """Runs whose key table is crowded: sub-batches of 64 ops give runs of at
+    most 64 ops and a key table of 128 entries, and the runs mutate up to 64
+    distinct keys of a wide key space (the all-ones key among them, which has
+    its own entry), so the table fills to half and probes wrap and run long.
+    Each batch checked in order, then the final contents."""
+    say("exec_batches_crowded_key_table", is_map, lanes)
+    rng = np.random.default_rng(6400 + lanes)
+    kb, vb = 8, 8
+    tbl = make(lslab, is_map, kb, vb, 6, 1 << 14, 1, lanes, subbatch=64, combine=1)
+    ref = Replay(is_map, kb, vb, 6, pool_slabs=1 << 14)
+    sizes = rng.integers(1, 12, size=3000).astype(np.int64)
+    sizes[[10, 200, 1999]] = 0
+    sizes[[50, 51, 900]] = [64, 63, 65]  # a full run, a near-full one, one above the cap
+    n = int(sizes.sum())
+    space = 6000
+    if is_map:
+        ranks = rng.integers(0, space, size=n)
+        hot = rng.random(n) < 0.3  # a hot share over few keys, so keys repeat across batches
+        ranks[hot] = rng.integers(0, 40, size=int(hot.sum()))
+    else:
+        ranks = np.concatenate([rng.permutation(space)[:s] for s in sizes])
+    keys = ranks.astype(np.uint64) * np.uint64(0x9E3779B97F4A7C15)
+    keys[ranks == 7] = np.uint64(0xFFFFFFFFFFFFFFFF)
+    ops = rng.choice([0, 1, 2], size=n, p=[0.4, 0.4, 0.2]).astype(np.uint8)
+    vals = rng.integers(1, 2**63, size=n, dtype=np.uint64)
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    d_offs = torch.from_numpy(offs).cuda()
+    d_ops = torch.from_numpy(ops).cuda()
+    if is_map:
+        f, v = tbl.exec_batches(d_offs, d_ops, to_dev(keys, kb), to_dev(vals, vb))
+        f, v = f.cpu().numpy(), to_host(v.contiguous(), vb)
+    else:
+        f = tbl.exec_batches(d_offs, d_ops, to_dev(keys, kb)).cpu().numpy()
+        v = np.zeros(n, np.uint64)
+    assert tbl.status() == 0
+    for b in range(len(sizes)):
+        lo, hi = offs[b], offs[b + 1]
+        if hi == lo:
+            continue
+        bad, first = ref.check_batch(ops[lo:hi], keys[lo:hi], vals[lo:hi], f[lo:hi], v[lo:hi])
+        assert bad == 0, (b, first)
+    k, vv = dump_sorted(tbl, is_map, kb, vb)
+    rk, rv = oracle_dump_sorted(ref)
+    np.testing.assert_array_equal(k, rk)
+    if is_map:
+        np.testing.assert_array_equal(vv, rv)
+
+
 @pytest.mark.parametrize("combine", [1, 2])
 def test_exec_batches_reclaim_runs(lslab, oracle_mod, combine):
     """The batched kernel on a reclaim table: many small batches (runs sharing
