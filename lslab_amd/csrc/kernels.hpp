@@ -742,6 +742,11 @@ struct LaunchCtx {
     uint64_t submax;   // two-phase ops per sub-batch (scratch capacity)
     uint64_t run_cap;  // ops a run of batches may hold (run scratch capacity)
     int num_cus;
+    // Recorded by the call's last kernel launch itself when the launch
+    // completes (a completion event of the dispatch: no separate marker in
+    // the stream); nullptr: none (graph capture, or calls that record their
+    // own).
+    hipEvent_t done_ev;
 };
 
 // Ops a run of the batched kernel may hold: the sub-batch size, at most 2^20

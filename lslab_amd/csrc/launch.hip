@@ -4,6 +4,8 @@
 #include <mutex>
 #include <unordered_map>
 
+#include <hip/hip_ext.h>
+
 #include "kernels.hpp"
 
 #ifndef LSLAB_LAYOUT
@@ -91,15 +93,15 @@ int Launch<L>::ops(const LaunchCtx& c, const OpsDev& o, const OutDev& out, uint6
     return dispatch_g(c.tun.lanes_per_op, [&](auto gc) {
         constexpr int G = decltype(gc)::value;
         if (read_only) {
-            hipLaunchKernelGGL((k_plain<L, G, false, false>), dim3((unsigned)grid_for(c, n)),
-                               dim3(bt), 0, s, c.dev, o, out, n);
+            hipExtLaunchKernelGGL((k_plain<L, G, false, false>), dim3((unsigned)grid_for(c, n)),
+                                  dim3(bt), 0, s, nullptr, c.done_ev, 0, c.dev, o, out, n);
             return launched();
         }
         return dispatch_r(c.tun.reclaim, [&](auto rr) {
             constexpr bool RC = decltype(rr)::value;
             if (!two_phase) {
-                hipLaunchKernelGGL((k_plain<L, G, true, RC>), dim3((unsigned)grid_for(c, n)),
-                                   dim3(bt), 0, s, c.dev, o, out, n);
+                hipExtLaunchKernelGGL((k_plain<L, G, true, RC>), dim3((unsigned)grid_for(c, n)),
+                                      dim3(bt), 0, s, nullptr, c.done_ev, 0, c.dev, o, out, n);
                 return launched();
             }
             return dispatch_c(c.tun.combine, [&](auto cc) {
@@ -109,8 +111,9 @@ int Launch<L>::ops(const LaunchCtx& c, const OpsDev& o, const OutDev& out, uint6
                     const uint64_t grid = grid_for(c, m);
                     hipLaunchKernelGGL((k_phase_r<L, G, BC>), dim3((unsigned)grid), dim3(bt), 0,
                                        s, c.dev, o, out, lo, m);
-                    hipLaunchKernelGGL((k_phase_w<L, G, RC>), dim3((unsigned)grid), dim3(bt), 0,
-                                       s, c.dev, o, lo, m);
+                    hipExtLaunchKernelGGL((k_phase_w<L, G, RC>), dim3((unsigned)grid), dim3(bt),
+                                          0, s, nullptr, lo + m >= n ? c.done_ev : nullptr, 0,
+                                          c.dev, o, lo, m);
                 }
                 return launched();
             });
@@ -144,13 +147,15 @@ int Launch<L>::batches(const LaunchCtx& c, const OpsDev& o, const OutDev& out,
                 hipLaunchKernelGGL(k_reset_batches<L>, dim3(1), dim3(1024), 0, s, bar, offsets,
                                    nbatches, c.run_cap, mode);
                 if (alone)
-                    hipLaunchKernelGGL((k_batches<L, G, BC, RC, false>), dim3((unsigned)grid),
-                                       dim3(PERSIST_THREADS), 0, s, c.dev, o, out, offsets,
-                                       nbatches, c.submax, c.run_cap, bar);
+                    hipExtLaunchKernelGGL((k_batches<L, G, BC, RC, false>), dim3((unsigned)grid),
+                                          dim3(PERSIST_THREADS), 0, s, nullptr,
+                                          runs ? nullptr : c.done_ev, 0, c.dev, o, out, offsets,
+                                          nbatches, c.submax, c.run_cap, bar);
                 if (runs)
-                    hipLaunchKernelGGL((k_batches<L, G, BC, RC, true>), dim3((unsigned)grid_r),
-                                       dim3(PERSIST_THREADS), 0, s, c.dev, o, out, offsets,
-                                       nbatches, c.submax, c.run_cap, bar);
+                    hipExtLaunchKernelGGL((k_batches<L, G, BC, RC, true>), dim3((unsigned)grid_r),
+                                          dim3(PERSIST_THREADS), 0, s, nullptr, c.done_ev, 0,
+                                          c.dev, o, out, offsets, nbatches, c.submax, c.run_cap,
+                                          bar);
                 return launched();
             });
         });
@@ -171,9 +176,9 @@ int Launch<L>::serve(const LaunchCtx& c, const OpsDev& o, const OutDev& out, con
                     c, reinterpret_cast<const void*>(k_serve<L, G, BC, RC>), &grid);
                 if (rc) return rc;
                 hipLaunchKernelGGL(k_reset_serve<L>, dim3(1), dim3(64), 0, s, ss, bar);
-                hipLaunchKernelGGL((k_serve<L, G, BC, RC>), dim3((unsigned)grid),
-                                   dim3(PERSIST_THREADS), 0, s, c.dev, o, out, sv, ss, bar,
-                                   c.submax);
+                hipExtLaunchKernelGGL((k_serve<L, G, BC, RC>), dim3((unsigned)grid),
+                                      dim3(PERSIST_THREADS), 0, s, nullptr, c.done_ev, 0, c.dev,
+                                      o, out, sv, ss, bar, c.submax);
                 return launched();
             });
         });

@@ -28,6 +28,7 @@
 // tag = low 32 bits of h.  Bucket = the key's first 8 bytes (identity, as the
 // narrow tables) or h >> 32 (LSLAB_HASH_MIX), masked to 2^n buckets.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <new>
@@ -447,22 +448,27 @@ int run(lslab_wide* w, const WideOps& o, uint64_t n, bool locked, hipStream_t s)
     if (n == 0) return LSLAB_OK;
     if (n > (1ull << 31) || !aligned4(o.keys) || !aligned4(o.vals) || !aligned4(o.out_vals))
         return LSLAB_ERR_INVALID;
-    if (!capturing(s)) WCHK(hipStreamWaitEvent(s, w->order_ev, 0));
+    // The call's last kernel launch records the order event on completion
+    // (no separate marker in the stream); none while s is being captured.
+    hipEvent_t done = nullptr;
+    if (!capturing(s)) {
+        WCHK(hipStreamWaitEvent(s, w->order_ev, 0));
+        done = w->order_ev;
+    }
     const uint64_t blocks = (n * WG + 255) / 256;
     if (locked && o.ops) {
         hipLaunchKernelGGL(k_wide<false>, dim3((unsigned)blocks), dim3(256), 0, s, w->dev, o, n,
                            (int)SEL_GETS);
-        hipLaunchKernelGGL(k_wide_muts, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, w->dev,
-                           o, n);
+        hipExtLaunchKernelGGL(k_wide_muts, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                              nullptr, done, 0, w->dev, o, n);
     } else if (locked) {
-        hipLaunchKernelGGL(k_wide<true>, dim3((unsigned)blocks), dim3(256), 0, s, w->dev, o, n,
-                           (int)SEL_ALL);
+        hipExtLaunchKernelGGL(k_wide<true>, dim3((unsigned)blocks), dim3(256), 0, s, nullptr, done,
+                              0, w->dev, o, n, (int)SEL_ALL);
     } else {
-        hipLaunchKernelGGL(k_wide<false>, dim3((unsigned)blocks), dim3(256), 0, s, w->dev, o, n,
-                           (int)SEL_ALL);
+        hipExtLaunchKernelGGL(k_wide<false>, dim3((unsigned)blocks), dim3(256), 0, s, nullptr,
+                              done, 0, w->dev, o, n, (int)SEL_ALL);
     }
     WCHK(hipGetLastError());
-    if (!capturing(s)) WCHK(hipEventRecord(w->order_ev, s));
     return LSLAB_OK;
 }
 
