@@ -256,7 +256,10 @@ typedef struct {
     const uint32_t* req_reserve;   /* nreq, or NULL */
     uint64_t* complete_ticks;      /* nreq, written */
     uint32_t* tick_of;             /* nreq, written */
-    uint64_t max_tick_ops;         /* 0 = the table's max_subbatch */
+    uint64_t max_tick_ops;         /* 0 = the table's max_subbatch; with a grid of
+                                      B > 1 blocks, one of which only admits, a
+                                      tick holds at most max_tick_ops * (B-1) / B
+                                      ops (the workers' share) */
     uint32_t max_tick_reqs;        /* 0 = no limit */
 } lslab_serve_params;
 

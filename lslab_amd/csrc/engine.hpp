@@ -752,7 +752,7 @@ __device__ __forceinline__ void phase_r_range(const TableDev& t, const OpsDev& o
     CombTab<BT>* tabs = nullptr;
     uint32_t* rcnt = rep_counter();
     if constexpr (BC) {
-        const uint64_t gid0 = (uint64_t)blockIdx.x * blockDim.x / G;
+        const uint64_t gid0 = gid - threadIdx.x / G;  // the block's first group
         if (gid0 >= m) return;  // block-uniform
         rounds = (m - gid0 + ng - 1) / ng;
         tabs = comb_tables<BT>();

@@ -354,14 +354,14 @@ __global__ void k_reset_batches(BarState* bar, const uint64_t* offsets, uint64_t
 //
 // Requests r = 0..nreq-1 own the contiguous op ranges [off[r], off[r+1]) and
 // arrive at arrival[r] (ticks of the 100 MHz constant clock after the loop
-// starts, nondecreasing).  Each tick, one scheduler lane admits, in arrival
-// order, every request that has arrived, up to max_tick_ops ops and
-// max_tick_reqs requests, provided the slab pool keeps headroom for it: a
-// request reserves one slab per put, the most its puts can allocate (an
-// insert allocates at most one slab).  A request whose reservation can never
-// fit (the pool only shrinks) is rejected.  All blocks then run the admitted
-// ops as one two-phase batch; the scheduler stamps each request's completion
-// time.  Requests arriving while a tick runs join the next tick.
+// starts, nondecreasing).  Each tick admits, in arrival order, every request
+// that has arrived, up to max_tick_ops ops and max_tick_reqs requests,
+// provided the slab pool keeps headroom for it: a request reserves one slab
+// per put, the most its puts can allocate (an insert allocates at most one
+// slab).  A request whose reservation can never fit (the pool only shrinks)
+// is rejected.  The other blocks run the admitted ops as one two-phase batch
+// while block 0 admits the next tick; the last block stamps each request's
+// completion time.  Requests arriving while a tick runs join the next tick.
 // ---------------------------------------------------------------------------
 
 struct ServeDev {
@@ -414,23 +414,28 @@ __device__ __forceinline__ uint64_t block_scan(uint64_t v, uint64_t* total) {
     return v + before;
 }
 
-// The next tick's admission, by all threads of block 0.  Thread 0 owns the
-// loop state.  It first finds the tick's first request, alone: it waits for
-// the request's arrival (an idle loop), or rejects it when the pool could
-// never hold its reservation.  Then the block takes the following requests in
-// chunks of one request per thread.  Each thread loads its request's arrival,
-// size and reservation; prefix sums give each request's running op and slab
-// totals; the tick admits the longest prefix whose requests have arrived and
-// fit the op cap, the request cap and the pool headroom.  These are the
-// rules of a request-by-request admission in arrival order, with the clock
-// read once per chunk.
+// A tick's admission, by all threads of block 0, in arrival order: the
+// requests that have arrived and fit the op cap, the request cap and the
+// slab-pool headroom, the rules of a request-by-request admission with the
+// clock read once per chunk.  The first request of a tick is taken by thread
+// 0 alone: it may wait for the request's arrival (an idle loop), or reject it
+// when the pool could never hold its reservation.  The following requests
+// are taken by the whole block in chunks of one request per thread: each
+// thread loads its request's arrival, size and reservation, prefix sums give
+// the running op and slab totals, and the tick admits the longest prefix
+// that passes every check.
 // Block 0's scheduler state for the whole launch (its LDS): the next request
-// to consider, and an upper bound of the slabs in use once every admitted
-// tick has finished (the last exact count plus the reservations admitted
-// since).
+// to consider, an upper bound of the slabs in use once every admitted tick
+// has finished (the last exact count plus the reservations admitted since),
+// and the admission being built.
 struct SchedState {
     uint64_t r;
     uint64_t bound;
+    uint64_t used;  // slabs in use the admission counts on
+    uint64_t lo;    // first request of the admission
+    uint64_t n, ops, res;
+    uint64_t off0;  // op offset of the admission's first request
+    int open;       // the admission stopped only at a request not yet arrived
 };
 
 __device__ __forceinline__ SchedState* sched_state() {
@@ -438,71 +443,85 @@ __device__ __forceinline__ SchedState* sched_state() {
     return &ss;
 }
 
-// exact: no admitted tick is still writing back, so the pool's counters are
-// final and are read; otherwise (block 0 schedules the next tick while the
-// other blocks finish the current tick's phase W) the headroom check uses
-// the bound, and a request the bound refuses is not admitted yet: it waits
-// for a later schedule, at the latest the exact one that an empty tick
-// leads to.  A request whose reservation exceeds the pool's exact headroom
-// can never fit (the pool only shrinks) and is rejected.
-__device__ inline void serve_schedule(const TableDev& t, const ServeDev& sv, ServeState* ss,
-                                      BarState* bar, bool exact) {
-    __shared__ uint64_t st_r, st_lo, st_n, st_ops, st_res, st_used, st_el, st_off;
-    __shared__ uint64_t st_pos[PERSIST_THREADS / 64];
-    __shared__ uint64_t st_ps, st_pn;
-    __shared__ int st_go;
+// Starts an empty admission.  exact: no admitted tick is still writing back,
+// so the pool's counters are final and are read; otherwise the headroom
+// check uses the bound, and a request the bound refuses is not admitted yet:
+// it waits for a later admission, at the latest the exact one that an empty
+// tick leads to.
+__device__ inline void sched_begin(const TableDev& t, bool exact) {
     SchedState* S = sched_state();
     if (threadIdx.x == 0) {
-        uint64_t r = S->r;
-        const uint64_t used = exact ? ld_agent(t.alloc_ctr) - ld_agent(t.free_count) : S->bound;
-        uint64_t lo = r, n = 0, ops = 0, reserved = 0, off0 = 0;
-        while (r < sv.nreq) {
-            if (sv.arrival[r] > now_ticks() - ss->t0) {
-                // Idle: wait for the next arrival, however far away, with a
-                // heartbeat every millisecond so that the blocks waiting in
-                // the grid barrier meanwhile do not take the loop for stuck.
-                uint64_t last = now_ticks();
-                for (;;) {
-                    const uint64_t now = now_ticks();
-                    if (now - ss->t0 >= sv.arrival[r]) break;
-                    if (now - last >= TICKS_PER_MS) {
-                        last = now;
-                        __hip_atomic_fetch_add(&bar->beat, 1u, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_AGENT);
+        S->used = exact ? ld_agent(t.alloc_ctr) - ld_agent(t.free_count) : S->bound;
+        S->lo = S->r;
+        S->n = S->ops = S->res = S->off0 = 0;
+        S->open = 1;
+    }
+    __syncthreads();
+}
+
+// Adds to the admission the requests that qualify now.  wait: with nothing
+// admitted yet, wait for the next request's arrival, however far away.  A
+// request whose reservation exceeds the pool's exact headroom can never fit
+// (the pool only shrinks) and is rejected.
+__device__ inline void sched_fill(const TableDev& t, const ServeDev& sv, ServeState* ss,
+                                  BarState* bar, bool exact, bool wait) {
+    __shared__ uint64_t st_el, st_ps, st_pn;
+    __shared__ uint64_t st_pos[PERSIST_THREADS / 64];
+    __shared__ int st_go;
+    SchedState* S = sched_state();
+    if (S->n == 0) {  // block-uniform (LDS, written before a block barrier)
+        if (threadIdx.x == 0) {
+            uint64_t r = S->r;
+            while (r < sv.nreq) {
+                if (sv.arrival[r] > now_ticks() - ss->t0) {
+                    if (!wait) break;
+                    // Idle: wait for the next arrival with a heartbeat every
+                    // millisecond, so that the blocks waiting in the grid
+                    // barrier meanwhile do not take the loop for stuck.
+                    uint64_t last = now_ticks();
+                    for (;;) {
+                        const uint64_t now = now_ticks();
+                        if (now - ss->t0 >= sv.arrival[r]) break;
+                        if (now - last >= TICKS_PER_MS) {
+                            last = now;
+                            __hip_atomic_fetch_add(&bar->beat, 1u, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+                        }
+                        __builtin_amdgcn_s_sleep(8);
                     }
-                    __builtin_amdgcn_s_sleep(8);
                 }
-            }
-            off0 = sv.off[r];
-            const uint64_t sz = sv.off[r + 1] - off0;
-            const uint64_t need = sv.reserve ? sv.reserve[r] : sz;
-            if (used + need > t.pool && !exact) break;  // wait for the exact count
-            if (used + need > t.pool) {  // can never fit: the pool only shrinks
-                sv.tick_of[r] = SERVE_REJECTED;
-                sv.complete[r] = now_ticks() - ss->t0;
-                ss->rejected++;
+                const uint64_t off0 = sv.off[r];
+                const uint64_t sz = sv.off[r + 1] - off0;
+                const uint64_t need = sv.reserve ? sv.reserve[r] : sz;
+                if (S->used + need > t.pool && !exact) {  // wait for the exact count
+                    S->open = 0;
+                    break;
+                }
+                if (S->used + need > t.pool) {  // can never fit: the pool only shrinks
+                    sv.tick_of[r] = SERVE_REJECTED;
+                    sv.complete[r] = now_ticks() - ss->t0;
+                    ss->rejected++;
+                    r++;
+                    S->lo = r;
+                    continue;
+                }
+                S->n = 1;
+                S->ops = sz;
+                S->res = need;
+                S->off0 = off0;
                 r++;
-                lo = r;
-                continue;
+                break;
             }
-            n = 1;
-            ops = sz;
-            reserved = need;
-            r++;
-            break;
+            if (r >= sv.nreq) S->open = 0;
+            S->r = r;
+            st_go = S->n > 0 && S->open;
         }
-        st_r = r;
-        st_lo = lo;
-        st_n = n;
-        st_ops = ops;
-        st_res = reserved;
-        st_used = used;
-        st_off = off0;
-        st_go = n > 0 && r < sv.nreq;
+    } else if (threadIdx.x == 0) {
+        st_go = S->open;
     }
     __syncthreads();
     while (st_go) {
-        const uint64_t r = st_r;
+        const uint64_t r = S->r;
         if (threadIdx.x == 0) st_el = now_ticks() - ss->t0;
         __syncthreads();
         const uint64_t i = r + threadIdx.x;
@@ -517,9 +536,10 @@ __device__ inline void serve_schedule(const TableDev& t, const ServeDev& sv, Ser
         uint64_t tot;
         const uint64_t ps = block_scan(sz, &tot);
         const uint64_t pn = block_scan(need, &tot);
-        const bool ok = valid && arrived && st_ops + ps <= sv.max_tick_ops &&
-                        st_n + threadIdx.x + 1 <= (uint64_t)sv.max_tick_reqs &&
-                        st_used + st_res + pn <= t.pool;
+        const bool fits = valid && S->ops + ps <= sv.max_tick_ops &&
+                          S->n + threadIdx.x + 1 <= (uint64_t)sv.max_tick_reqs &&
+                          S->used + S->res + pn <= t.pool;
+        const bool ok = fits && arrived;
         // the first request that fails: the tick admits the ones before it
         uint64_t f = ok ? (uint64_t)blockDim.x : (uint64_t)threadIdx.x;
         for (int d = 32; d > 0; d >>= 1) {
@@ -534,44 +554,64 @@ __device__ inline void serve_schedule(const TableDev& t, const ServeDev& sv, Ser
             st_ps = ps;
             st_pn = pn;
         }
+        // a later fill can only add requests when the first failure was an
+        // arrival still to come
+        if (k < blockDim.x && threadIdx.x == k) S->open = fits && !arrived;
         __syncthreads();
         if (threadIdx.x == 0) {
             if (k > 0) {
-                st_n += k;
-                st_ops += st_ps;
-                st_res += st_pn;
-                st_r = r + k;
+                S->n += k;
+                S->ops += st_ps;
+                S->res += st_pn;
+                S->r = r + k;
             }
-            st_go = k == blockDim.x && st_r < sv.nreq;
+            if (S->r >= sv.nreq) S->open = 0;
+            st_go = k == blockDim.x && S->open;
         }
         __syncthreads();
     }
+}
+
+// Publishes the admission as the next tick (read by every block after the
+// grid barrier: agent-scope stores).  The admitted requests are contiguous,
+// so their ops are [off[r_lo], off[r_lo] + ops).  No admission with requests
+// left is an empty tick; with none left the loop is done.
+__device__ inline void sched_publish(const ServeDev& sv, ServeState* ss) {
+    SchedState* S = sched_state();
     if (threadIdx.x == 0) {
-        // Read by every block after the barrier: agent-scope stores.  The
-        // admitted requests are contiguous, so their ops are
-        // [off[r_lo], off[r_lo] + ops).  No admission with requests left is
-        // an empty tick; with none left the loop is done.
-        const uint64_t op_lo = st_n ? st_off : 0;
-        st_coh(&ss->r_lo, st_lo);
-        st_coh(&ss->r_hi, st_r);
+        const uint64_t op_lo = S->n ? S->off0 : 0;
+        st_coh(&ss->r_lo, S->lo);
+        st_coh(&ss->r_hi, S->r);
         st_coh(&ss->op_lo, op_lo);
-        st_coh(&ss->op_hi, op_lo + st_ops);
-        ss->next_req = st_r;
-        ss->served_ops += st_ops;
-        if (st_n == 0 && st_r >= sv.nreq) st_coh(&ss->done, 1u);
-        S->r = st_r;
-        S->bound = st_used + st_res;  // the admitted tick allocates at most its reservations
+        st_coh(&ss->op_hi, op_lo + S->ops);
+        ss->next_req = S->r;
+        ss->served_ops += S->ops;
+        if (S->n == 0 && S->r >= sv.nreq) st_coh(&ss->done, 1u);
+        S->bound = S->used + S->res;  // the admitted tick allocates at most its reservations
     }
 }
 
+// Served loop.  With more than one block, block 0 only schedules: it builds
+// the next tick's admission while the other blocks run the current tick's
+// phase R, adds the requests that arrived meanwhile while they run phase W,
+// and publishes it at the grid barrier that ends the tick, so admission is
+// off the tick's critical path.  Each tick: phase R, grid barrier,
+// completion stamps (last block), phase W, grid barrier; a tick larger than
+// a sub-batch runs in sub-batches, with a barrier between.  After an empty
+// tick (nothing admitted) block 0 admits with the pool's exact counts,
+// waiting for the next arrival if there is none.
 template <class L, int G, bool BC, bool RC>
 __global__ __launch_bounds__(PERSIST_THREADS) void k_serve(TableDev t, OpsDev o, OutDev out, ServeDev sv,
                                                ServeState* ss, BarState* bar, uint64_t submax) {
-    const uint64_t ng = (uint64_t)gridDim.x * blockDim.x / G;
-    const uint64_t gid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+    const bool split = gridDim.x > 1;  // block 0 schedules only
+    const uint64_t wb = split ? blockIdx.x - (blockIdx.x > 0) : blockIdx.x;  // worker index
+    const uint64_t nwb = split ? gridDim.x - 1 : gridDim.x;
+    const uint64_t ng = nwb * blockDim.x / G;
+    const uint64_t gid = (wb * blockDim.x + threadIdx.x) / G;
     __shared__ uint64_t stamp_s;
     uint32_t* rc = rep_counter();
-    RepEnt* rb = t.rep + block_rep_base_fixed(submax, ng, blockDim.x / G, blockIdx.x);
+    RepEnt* rb = t.rep + block_rep_base_fixed(submax, ng, blockDim.x / G, wb);
+    SchedState* S = sched_state();
     if (threadIdx.x == 0) *rc = 0;
     if constexpr (BC)
         comb_init<PERSIST_THREADS>();
@@ -579,49 +619,81 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_serve(TableDev t, OpsDev o,
         __syncthreads();
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         st_coh(&ss->t0, now_ticks());  // read by the stamping block
-        sched_state()->r = 0;
-        sched_state()->bound = 0;
+        S->r = 0;
+        S->bound = 0;
     }
-    // Block 0 admits the next tick at the top of the loop.  After a tick it
-    // does so as soon as its own phase W is done, while the other blocks
-    // finish theirs: the grid barrier that ends the write-back also
-    // publishes the next tick (two grid barriers per tick).
-    bool running = false;  // a tick's phase W is still running on other blocks
-    uint64_t ntick = 0;    // ticks with ops
-    for (;;) {
-        if (blockIdx.x == 0) serve_schedule(t, sv, ss, bar, !running);  // block-uniform
-        if (!grid_sync(bar, gridDim.x, t.spin_ticks, running, rc)) goto fail;
-        running = false;
-        if (ld_agent(&ss->done)) break;
-        const uint64_t r_lo = ld_coh(&ss->r_lo), r_hi = ld_coh(&ss->r_hi);
-        const uint64_t lo = ld_coh(&ss->op_lo), hi = ld_coh(&ss->op_hi);
-        if (lo == hi) continue;  // empty tick: schedule again, exactly
-        for (uint64_t slo = lo;;) {
-            const uint64_t m = min(submax, hi - slo);
-            phase_r_range<L, G, PERSIST_THREADS, BC>(t, o, out, slo, m, gid, ng, rb, rep_lds(),
-                                                     REP_LDS);
-            if (!grid_sync(bar, gridDim.x, t.spin_ticks, false)) goto fail;
-            // Every result of the tick is written in phase R: its requests
-            // complete here.  Phase W only brings the table up to date for
-            // the next tick.
-            // The last block stamps them (block 0 has the scheduler's work).
-            if (slo + m >= hi && blockIdx.x == gridDim.x - 1) {
-                if (threadIdx.x == 0) stamp_s = now_ticks() - ld_coh(&ss->t0);
-                __syncthreads();
-                for (uint64_t r = r_lo + threadIdx.x; r < r_hi; r += blockDim.x) {
-                    sv.complete[r] = stamp_s;
-                    sv.tick_of[r] = (uint32_t)ntick;
-                }
-                __syncthreads();
-                if (threadIdx.x == 0) ss->ticks = ntick + 1;
+    if (split && blockIdx.x == 0) {
+        // The scheduling block: the same grid barriers as the workers, and
+        // admission in between.  A tick's op cap counts the workers' share
+        // of the grid, so that a tick takes as many phase-R rounds on the
+        // workers as the cap would on the whole grid.
+        sv.max_tick_ops = max(sv.max_tick_ops * (gridDim.x - 1) / gridDim.x, (uint64_t)1);
+        bool running = false;  // the last tick's admission is already built
+        for (;;) {
+            if (!running) {
+                sched_begin(t, true);
+                sched_fill(t, sv, ss, bar, true, true);
             }
-            phase_w_block<L, G, RC>(t, o, slo, rb, *rc, rep_lds(), REP_LDS);
-            slo += m;
-            if (slo >= hi) break;
-            if (!grid_sync(bar, gridDim.x, t.spin_ticks, true, rc)) goto fail;
+            sched_publish(sv, ss);
+            if (!grid_sync(bar, gridDim.x, t.spin_ticks, running, rc)) goto fail;
+            running = false;
+            if (ld_agent(&ss->done)) break;
+            const uint64_t lo = ld_coh(&ss->op_lo), hi = ld_coh(&ss->op_hi);
+            if (lo == hi) continue;  // empty tick: admit again, exactly
+            sched_begin(t, false);  // the next tick's admission, during this tick
+            sched_fill(t, sv, ss, bar, false, false);
+            for (uint64_t slo = lo;;) {
+                const uint64_t m = min(submax, hi - slo);
+                if (!grid_sync(bar, gridDim.x, t.spin_ticks, false)) goto fail;
+                if (slo + m >= hi)  // requests that arrived during the tick
+                    sched_fill(t, sv, ss, bar, false, false);
+                slo += m;
+                if (slo >= hi) break;
+                if (!grid_sync(bar, gridDim.x, t.spin_ticks, true, rc)) goto fail;
+            }
+            running = true;
         }
-        ++ntick;
-        running = true;
+    } else {
+        bool running = false;  // a tick wrote back since the last exact admission
+        uint64_t ntick = 0;    // ticks with ops
+        for (;;) {
+            if (!split) {  // one block: it admits each tick itself
+                sched_begin(t, !running);
+                sched_fill(t, sv, ss, bar, !running, true);
+                sched_publish(sv, ss);
+            }
+            if (!grid_sync(bar, gridDim.x, t.spin_ticks, running, rc)) goto fail;
+            running = false;
+            if (ld_agent(&ss->done)) break;
+            const uint64_t r_lo = ld_coh(&ss->r_lo), r_hi = ld_coh(&ss->r_hi);
+            const uint64_t lo = ld_coh(&ss->op_lo), hi = ld_coh(&ss->op_hi);
+            if (lo == hi) continue;  // empty tick: admit again, exactly
+            for (uint64_t slo = lo;;) {
+                const uint64_t m = min(submax, hi - slo);
+                phase_r_range<L, G, PERSIST_THREADS, BC>(t, o, out, slo, m, gid, ng, rb,
+                                                         rep_lds(), REP_LDS);
+                if (!grid_sync(bar, gridDim.x, t.spin_ticks, false)) goto fail;
+                // Every result of the tick is written in phase R: its
+                // requests complete here.  Phase W only brings the table up
+                // to date for the next tick.  The last block stamps them.
+                if (slo + m >= hi && blockIdx.x == gridDim.x - 1) {
+                    if (threadIdx.x == 0) stamp_s = now_ticks() - ld_coh(&ss->t0);
+                    __syncthreads();
+                    for (uint64_t r = r_lo + threadIdx.x; r < r_hi; r += blockDim.x) {
+                        sv.complete[r] = stamp_s;
+                        sv.tick_of[r] = (uint32_t)ntick;
+                    }
+                    __syncthreads();
+                    if (threadIdx.x == 0) ss->ticks = ntick + 1;
+                }
+                phase_w_block<L, G, RC>(t, o, slo, rb, *rc, rep_lds(), REP_LDS);
+                slo += m;
+                if (slo >= hi) break;
+                if (!grid_sync(bar, gridDim.x, t.spin_ticks, true, rc)) goto fail;
+            }
+            ++ntick;
+            running = true;
+        }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) ss->end = now_ticks() - ss->t0;
     return;

@@ -69,8 +69,11 @@ def test_alone_equals_mix(lslab, oracle_mod):
         np.testing.assert_array_equal(v[lo:hi], rv, err_msg=f"request {r}")
 
 
-def test_shared_hot_keys_serial_order(lslab, oracle_mod):
-    say("serve hot keys")
+@pytest.mark.parametrize("max_blocks", [0, 1, 2])
+def test_shared_hot_keys_serial_order(lslab, oracle_mod, max_blocks):
+    """Full grid (block 0 admits while the others run ticks), one block (it
+    admits each tick itself) and two blocks (one admitting, one running)."""
+    say("serve hot keys", max_blocks)
     rng = np.random.default_rng(9)
     nreq = 200
     sizes = rng.integers(1, 400, size=nreq)
@@ -82,7 +85,8 @@ def test_shared_hot_keys_serial_order(lslab, oracle_mod):
     keys = rng.choice(space, size=n, p=p).astype(np.uint64) * np.uint64(0x9E3779B97F4A7C15)
     ops = rng.choice([0, 1], size=n).astype(np.uint8)
     vals = rng.integers(1, 2**63, size=n, dtype=np.uint64)
-    m = lslab.Map(8, 8, 10, 1 << 14, lslab.tuning(engine=1, max_subbatch=1 << 13))
+    m = lslab.Map(8, 8, 10, 1 << 14, lslab.tuning(engine=1, max_subbatch=1 << 13,
+                                                  max_blocks=max_blocks))
     arrival = np.sort(rng.integers(0, 2000 * TICKS_PER_US, size=nreq))
     res = m.serve(_t(offs), _t(arrival), _t(ops, np.uint8), _t(keys.view(np.int64)),
                   _t(vals.view(np.int64)), max_tick_ops=1 << 13)
