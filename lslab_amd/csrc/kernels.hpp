@@ -591,6 +591,14 @@ __device__ inline void sched_publish(const ServeDev& sv, ServeState* ss) {
     }
 }
 
+// A wave-uniform 64-bit value moved to scalar registers (the tick's bounds,
+// loaded with atomic loads, would otherwise hold vector registers across
+// phase R).
+__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
+    return ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32)) << 32) |
+           __builtin_amdgcn_readfirstlane((uint32_t)v);
+}
+
 // Served loop.  With more than one block, block 0 only schedules: it builds
 // the next tick's admission while the other blocks run the current tick's
 // phase R, adds the requests that arrived meanwhile while they run phase W,
@@ -608,7 +616,7 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_serve(TableDev t, OpsDev o,
     const uint64_t nwb = split ? gridDim.x - 1 : gridDim.x;
     const uint64_t ng = nwb * blockDim.x / G;
     const uint64_t gid = (wb * blockDim.x + threadIdx.x) / G;
-    __shared__ uint64_t stamp_s;
+    __shared__ uint64_t stamp_s, tick_req[2];
     uint32_t* rc = rep_counter();
     RepEnt* rb = t.rep + block_rep_base_fixed(submax, ng, blockDim.x / G, wb);
     SchedState* S = sched_state();
@@ -665,8 +673,15 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_serve(TableDev t, OpsDev o,
             if (!grid_sync(bar, gridDim.x, t.spin_ticks, running, rc)) goto fail;
             running = false;
             if (ld_agent(&ss->done)) break;
-            const uint64_t r_lo = ld_coh(&ss->r_lo), r_hi = ld_coh(&ss->r_hi);
-            const uint64_t lo = ld_coh(&ss->op_lo), hi = ld_coh(&ss->op_hi);
+            // The tick's fields are read now: block 0 publishes the next
+            // tick's while this one writes back.  The stamping block keeps
+            // the tick's requests in LDS until phase R is done.
+            if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+                tick_req[0] = ld_coh(&ss->r_lo);
+                tick_req[1] = ld_coh(&ss->r_hi);
+            }
+            const uint64_t lo = uniform64(ld_coh(&ss->op_lo));
+            const uint64_t hi = uniform64(ld_coh(&ss->op_hi));
             if (lo == hi) continue;  // empty tick: admit again, exactly
             for (uint64_t slo = lo;;) {
                 const uint64_t m = min(submax, hi - slo);
@@ -679,7 +694,8 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_serve(TableDev t, OpsDev o,
                 if (slo + m >= hi && blockIdx.x == gridDim.x - 1) {
                     if (threadIdx.x == 0) stamp_s = now_ticks() - ld_coh(&ss->t0);
                     __syncthreads();
-                    for (uint64_t r = r_lo + threadIdx.x; r < r_hi; r += blockDim.x) {
+                    for (uint64_t r = tick_req[0] + threadIdx.x; r < tick_req[1];
+                         r += blockDim.x) {
                         sv.complete[r] = stamp_s;
                         sv.tick_of[r] = (uint32_t)ntick;
                     }
