@@ -19,6 +19,14 @@ using namespace lslab_dev;
 // keeps grid barriers cheap (256 arrivals, one cache invalidate per CU).
 constexpr int PERSIST_THREADS = 1024;
 
+// A wave-uniform 64-bit value moved to scalar registers (loop bounds loaded
+// from LDS or with atomic loads would otherwise hold vector registers across
+// the phases of a batch, run or tick).
+__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
+    return ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32)) << 32) |
+           __builtin_amdgcn_readfirstlane((uint32_t)v);
+}
+
 // Grid-wide barrier state for the persistent kernels.  Blocks arrive on one
 // of BAR_GROUPS group counters (block id mod BAR_GROUPS), the last block of a
 // group arrives on the top counter, and the last group releases every group
@@ -256,8 +264,8 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_batches(TableDev t, OpsDev 
         if (threadIdx.x < wn && ro->win[threadIdx.x + 1] - ro->win[0] > run_cap)
             atomicMin(&ro->nb, (int)threadIdx.x);
         __syncthreads();
-        const uint64_t lo = ro->win[0];
-        const uint64_t nr = (uint64_t)ro->nb;
+        const uint64_t lo = uniform64(ro->win[0]);
+        const uint64_t nr = uniform64((uint64_t)ro->nb);
         const uint64_t b2 = b + (nr <= 1 ? 1 : nr);  // the next run's first batch
         const uint64_t wn2 = b2 < nbatches ? min((uint64_t)RUN_MAX_BATCHES, nbatches - b2) : 0;
         const bool pfl = b2 < nbatches && threadIdx.x <= wn2;
@@ -268,7 +276,7 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_batches(TableDev t, OpsDev 
             if (threadIdx.x == 0) nx->nb = (int)wn2;
         };
         if (nr <= 1) {  // batch b alone, in sub-batches of at most submax ops
-            const uint64_t hi = ro->win[1];
+            const uint64_t hi = uniform64(ro->win[1]);
             if (pend) {
                 flush_w();
                 if (!grid_sync(bar, gridDim.x, t.spin_ticks, true, rc, &cc[par ^ 1])) return fail();
@@ -290,7 +298,7 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_batches(TableDev t, OpsDev 
             b = b2;
             continue;
         }
-        const uint64_t m = ro->win[nr] - lo;
+        const uint64_t m = uniform64(ro->win[nr] - lo);
         if (threadIdx.x <= nr) ro->off[threadIdx.x] = ro->win[threadIdx.x] - lo;
         __syncthreads();
         const RunTabs rt = par ? t.run[1] : t.run[0];
@@ -591,14 +599,6 @@ __device__ inline void sched_publish(const ServeDev& sv, ServeState* ss) {
     }
 }
 
-// A wave-uniform 64-bit value moved to scalar registers (the tick's bounds,
-// loaded with atomic loads, would otherwise hold vector registers across
-// phase R).
-__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
-    return ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32)) << 32) |
-           __builtin_amdgcn_readfirstlane((uint32_t)v);
-}
-
 // Served loop.  With more than one block, block 0 only schedules: it builds
 // the next tick's admission while the other blocks run the current tick's
 // phase R, adds the requests that arrived meanwhile while they run phase W,
@@ -635,7 +635,8 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_serve(TableDev t, OpsDev o,
         // admission in between.  A tick's op cap counts the workers' share
         // of the grid, so that a tick takes as many phase-R rounds on the
         // workers as the cap would on the whole grid.
-        sv.max_tick_ops = max(sv.max_tick_ops * (gridDim.x - 1) / gridDim.x, (uint64_t)1);
+        if (sv.max_tick_ops < (1ull << 48))  // larger caps do not bind
+            sv.max_tick_ops = max(sv.max_tick_ops * (gridDim.x - 1) / gridDim.x, (uint64_t)1);
         bool running = false;  // the last tick's admission is already built
         for (;;) {
             if (!running) {
