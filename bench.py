@@ -552,6 +552,15 @@ def reference_harness(lslab_amd, torch, reps=100):
         if m.status():
             raise RuntimeError("reference harness status")
         m.close()
+    # The same workload from C++ over the header-only API (tests/cpp/ref_harness,
+    # built with the library), without Python's per-call overhead.
+    exe = os.path.join(ROOT, "tests", "cpp", "ref_harness")
+    if os.path.exists(exe):
+        import subprocess
+        r = subprocess.run([exe, str(reps)], capture_output=True, text=True, timeout=120)
+        if r.returncode != 0:
+            raise RuntimeError("ref_harness: " + r.stderr[-400:])
+        res["cpp"] = json.loads(r.stdout.strip().splitlines()[-1])
     return res
 
 
