@@ -664,7 +664,18 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_serve(TableDev t, OpsDev o,
         }
     } else {
         bool running = false;  // a tick wrote back since the last exact admission
-        uint64_t ntick = 0;    // ticks with ops
+        uint64_t ntick = 0;    // ticks that admitted requests (counted by the stamping block)
+        // The stamping block: the tick's requests complete now.
+        auto stamp_tick = [&]() {
+            if (threadIdx.x == 0) stamp_s = now_ticks() - ld_coh(&ss->t0);
+            __syncthreads();
+            for (uint64_t r = tick_req[0] + threadIdx.x; r < tick_req[1]; r += blockDim.x) {
+                sv.complete[r] = stamp_s;
+                sv.tick_of[r] = (uint32_t)ntick;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) ss->ticks = ntick + 1;
+        };
         for (;;) {
             if (!split) {  // one block: it admits each tick itself
                 sched_begin(t, !running);
@@ -683,7 +694,16 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_serve(TableDev t, OpsDev o,
             }
             const uint64_t lo = uniform64(ld_coh(&ss->op_lo));
             const uint64_t hi = uniform64(ld_coh(&ss->op_hi));
-            if (lo == hi) continue;  // empty tick: admit again, exactly
+            if (lo == hi) {  // no ops: admit again, exactly
+                if (blockIdx.x == gridDim.x - 1) {
+                    __syncthreads();
+                    if (tick_req[0] < tick_req[1]) {  // requests of no ops complete at once
+                        stamp_tick();
+                        ++ntick;
+                    }
+                }
+                continue;
+            }
             for (uint64_t slo = lo;;) {
                 const uint64_t m = min(submax, hi - slo);
                 phase_r_range<L, G, PERSIST_THREADS, BC>(t, o, out, slo, m, gid, ng, rb,
@@ -692,17 +712,7 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_serve(TableDev t, OpsDev o,
                 // Every result of the tick is written in phase R: its
                 // requests complete here.  Phase W only brings the table up
                 // to date for the next tick.  The last block stamps them.
-                if (slo + m >= hi && blockIdx.x == gridDim.x - 1) {
-                    if (threadIdx.x == 0) stamp_s = now_ticks() - ld_coh(&ss->t0);
-                    __syncthreads();
-                    for (uint64_t r = tick_req[0] + threadIdx.x; r < tick_req[1];
-                         r += blockDim.x) {
-                        sv.complete[r] = stamp_s;
-                        sv.tick_of[r] = (uint32_t)ntick;
-                    }
-                    __syncthreads();
-                    if (threadIdx.x == 0) ss->ticks = ntick + 1;
-                }
+                if (slo + m >= hi && blockIdx.x == gridDim.x - 1) stamp_tick();
                 phase_w_block<L, G, RC>(t, o, slo, rb, *rc, rep_lds(), REP_LDS);
                 slo += m;
                 if (slo >= hi) break;

@@ -169,3 +169,35 @@ def test_idle_gaps_longer_than_the_spin_bound(lslab, oracle_mod):
     rf, rv, _ = ref.apply(ops, keys, vals)
     np.testing.assert_array_equal(res["found"].cpu().numpy(), rf)
     np.testing.assert_array_equal(res["value"].contiguous().cpu().numpy().view(np.uint64), rv)
+
+
+@pytest.mark.parametrize("max_blocks", [0, 1])
+def test_requests_without_ops(lslab, oracle_mod, max_blocks):
+    """Requests of no ops (equal consecutive offsets), alone in a tick and
+    beside others, are admitted and stamped like any other: every request
+    completes after its arrival with a tick id, and the ops of the others
+    give the replay's results."""
+    say("serve requests without ops", max_blocks)
+    sizes = np.array([5, 0, 0, 7, 0, 3, 0])
+    nreq = len(sizes)
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    n = int(offs[-1])
+    keys = np.arange(1, n + 1, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15)
+    ops = np.ones(n, np.uint8)
+    vals = keys + np.uint64(5)
+    # requests 1, 2 and 4 arrive alone; 6 arrives with 5
+    arrival = np.array([0, 100, 200, 300, 400, 500, 500], dtype=np.int64) * TICKS_PER_US
+    m = lslab.Map(8, 8, 8, 1 << 10, lslab.tuning(engine=1, max_subbatch=1 << 10,
+                                                 max_blocks=max_blocks))
+    res = m.serve(_t(offs), _t(arrival), _t(ops, np.uint8), _t(keys.view(np.int64)),
+                  _t(vals.view(np.int64)), max_tick_ops=1 << 10)
+    assert m.status() == 0
+    assert res["rejected"] == 0 and res["served_ops"] == n
+    comp = res["complete_ticks"].cpu().numpy()
+    assert (comp >= arrival).all(), comp
+    tick_of = res["tick_of"].cpu().numpy()
+    assert (np.diff(tick_of) >= 0).all() and tick_of[-1] < res["ticks"]
+    ref = Replay(True, 8, 8, 8, pool_slabs=1 << 10)
+    rf, rv, _ = ref.apply(ops, keys, vals)
+    np.testing.assert_array_equal(res["found"].cpu().numpy(), rf)
+    np.testing.assert_array_equal(res["value"].contiguous().cpu().numpy().view(np.uint64), rv)
