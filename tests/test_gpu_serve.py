@@ -201,3 +201,40 @@ def test_requests_without_ops(lslab, oracle_mod, max_blocks):
     rf, rv, _ = ref.apply(ops, keys, vals)
     np.testing.assert_array_equal(res["found"].cpu().numpy(), rf)
     np.testing.assert_array_equal(res["value"].contiguous().cpu().numpy().view(np.uint64), rv)
+
+
+@pytest.mark.parametrize("max_blocks", [0, 1])
+def test_requests_larger_than_a_sub_batch(lslab, oracle_mod, max_blocks):
+    """A request larger than the tick cap is admitted alone, and a tick larger
+    than the table's sub-batch runs in sub-batches (phase R and W each, with
+    the grid barriers between): results are the replay's (distinct keys) and
+    every request completes after its arrival."""
+    say("serve requests larger than a sub-batch", max_blocks)
+    rng = np.random.default_rng(77)
+    sizes = np.array([3000, 10, 2500, 700, 900])
+    nreq = len(sizes)
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    n = int(offs[-1])
+    keys = rng.permutation(1 << 16)[:n].astype(np.uint64) * np.uint64(0x9E3779B97F4A7C15)
+    ops = rng.choice([0, 1, 2], size=n, p=[0.3, 0.6, 0.1]).astype(np.uint8)
+    vals = rng.integers(1, 2**63, size=n, dtype=np.uint64)
+    arrival = np.zeros(nreq, dtype=np.int64)
+    m = lslab.Map(8, 8, 8, 1 << 12, lslab.tuning(engine=1, max_subbatch=1 << 10,
+                                                 max_blocks=max_blocks))
+    res = m.serve(_t(offs), _t(arrival), _t(ops, np.uint8), _t(keys.view(np.int64)),
+                  _t(vals.view(np.int64)), max_tick_ops=2048)
+    assert m.status() == 0
+    assert res["rejected"] == 0 and res["served_ops"] == n
+    tick_of = res["tick_of"].cpu().numpy()
+    assert tick_of[0] != tick_of[2]  # each over the cap: alone in its tick
+    ref = Replay(True, 8, 8, 8, pool_slabs=1 << 12)
+    rf, rv, _ = ref.apply(ops, keys, vals)
+    np.testing.assert_array_equal(res["found"].cpu().numpy(), rf)
+    np.testing.assert_array_equal(res["value"].contiguous().cpu().numpy().view(np.uint64), rv)
+    k, vv = m.dump()
+    rk, rvv = ref.dump()
+    k = k.cpu().numpy().view(np.uint64)
+    vv = vv.cpu().numpy().view(np.uint64)
+    o1, o2 = np.argsort(k), np.argsort(rk)
+    np.testing.assert_array_equal(k[o1], rk[o2])
+    np.testing.assert_array_equal(vv[o1], rvv[o2])
