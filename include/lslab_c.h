@@ -212,7 +212,9 @@ int lslab_set_mixed(lslab_table* t, const uint8_t* ops, const void* keys, uint8_
  * the two-phase engine (batch b sees every effect of batches < b; per key a
  * batch's gets come before its mutations), so the
  * call is one graph node when captured.  out as lslab_map_mixed (maps) or
- * bool per op (sets, values ignored). */
+ * bool per op (sets, values ignored).  Offsets must not decrease: the device
+ * checks them first and refuses the whole call otherwise (no op runs; the
+ * next lslab_status returns LSLAB_ERR_INVALID). */
 int lslab_exec_batches(lslab_table* t, size_t nbatches, const uint64_t* offsets,
                        const uint8_t* ops, const void* keys, const void* values, void* out,
                        hipStream_t stream);
@@ -247,7 +249,10 @@ int lslab_exec_batches_noalloc(lslab_table* t, size_t nbatches, const uint64_t* 
  * slabs -- one per put bounds what a request can allocate; NULL reserves one
  * per op), runs each tick as one two-phase batch, and writes
  * complete_ticks[r] (same clock) and tick_of[r] (LSLAB_SERVE_REJECTED when the
- * pool can never hold the request).  All arrays are DEVICE memory. */
+ * pool can never hold the request).  All arrays are DEVICE memory.  Request
+ * offsets must not decrease: the loop ends at the first request whose end
+ * lies before its start (it and later requests do not run; the next
+ * lslab_status returns LSLAB_ERR_INVALID). */
 #define LSLAB_SERVE_REJECTED 0xffffffffu
 typedef struct {
     uint64_t nreq;

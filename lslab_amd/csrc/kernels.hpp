@@ -47,7 +47,8 @@ struct BarState {
     uint32_t abort;
     uint32_t beat;  // heartbeat of a live wait elsewhere (the served loop's idle scheduler)
     uint32_t runs;  // batched launch: 1 if two consecutive batches fit one run (k_reset_batches)
-    uint32_t pad[29];
+    uint32_t bad;   // batched launch: the batch offsets decrease somewhere (k_reset_batches)
+    uint32_t pad[28];
 };
 
 // Plain per-batch kernel: one G-lane group per op, grid-stride.  RC: the
@@ -191,6 +192,7 @@ __global__ __launch_bounds__(PERSIST_THREADS) void k_batches(TableDev t, OpsDev 
                                                  const uint64_t* offsets, uint64_t nbatches,
                                                  uint64_t submax, uint64_t run_cap,
                                                  BarState* bar) {
+    if (ld_agent(&bar->bad)) return;                   // grid-uniform: refused input
     if ((ld_agent(&bar->runs) != 0) != RUNS) return;  // grid-uniform
     const uint64_t ng = (uint64_t)gridDim.x * blockDim.x / G;
     const uint64_t gid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
@@ -339,22 +341,35 @@ __global__ void k_reset_bar(BarState* bar) { reset_bar(bar); }
 // Before a batched launch (one block): resets the barrier and sets
 // bar->runs.  With mode LSLAB_RUNS_ON or _OFF that is 1 or 0; with 0 (auto)
 // it is whether any two consecutive batches together hold at most run_cap
-// ops, i.e. whether k_batches would form a run.
+// ops, i.e. whether k_batches would form a run.  Sets bar->bad (and the
+// table's ST_BAD_INPUT) when the offsets decrease.
 template <class L>
 __global__ void k_reset_batches(BarState* bar, const uint64_t* offsets, uint64_t nbatches,
-                                uint64_t run_cap, int mode) {
+                                uint64_t run_cap, int mode, uint32_t* status) {
     reset_bar(bar);
-    __shared__ uint32_t any;
-    if (threadIdx.x == 0) any = mode == LSLAB_RUNS_ON ? 1u : 0u;
-    __syncthreads();
-    if (mode == 0) {
-        bool fits = false;
-        for (uint64_t i = threadIdx.x; i + 2 <= nbatches; i += blockDim.x)
-            fits |= offsets[i + 2] - offsets[i] <= run_cap;
-        if (fits) any = 1;
-        __syncthreads();
+    __shared__ uint32_t any, bad;
+    if (threadIdx.x == 0) {
+        any = mode == LSLAB_RUNS_ON ? 1u : 0u;
+        bad = 0;
     }
-    if (threadIdx.x == 0) st_coh(&bar->runs, any);
+    __syncthreads();
+    // Decreasing offsets would give a batch a negative (wrapped) size: the
+    // launch is refused before any op is read.
+    bool fits = false, dec = false;
+    for (uint64_t i = threadIdx.x; i < nbatches; i += blockDim.x) {
+        const uint64_t a = offsets[i];
+        dec |= offsets[i + 1] < a;
+        if (mode == 0 && i + 2 <= nbatches) fits |= offsets[i + 2] - a <= run_cap;
+    }
+    if (fits) any = 1;
+    if (dec) bad = 1;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        st_coh(&bar->runs, any);
+        st_coh(&bar->bad, bad);
+        if (bad)
+            __hip_atomic_fetch_or(status, ST_BAD_INPUT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -444,6 +459,7 @@ struct SchedState {
     uint64_t n, ops, res;
     uint64_t off0;  // op offset of the admission's first request
     int open;       // the admission stopped only at a request not yet arrived
+    int bad;        // request offsets decrease at request r: the loop ends there
 };
 
 __device__ __forceinline__ SchedState* sched_state() {
@@ -463,6 +479,7 @@ __device__ inline void sched_begin(const TableDev& t, bool exact) {
         S->lo = S->r;
         S->n = S->ops = S->res = S->off0 = 0;
         S->open = 1;
+        S->bad = 0;
     }
     __syncthreads();
 }
@@ -499,7 +516,15 @@ __device__ inline void sched_fill(const TableDev& t, const ServeDev& sv, ServeSt
                     }
                 }
                 const uint64_t off0 = sv.off[r];
-                const uint64_t sz = sv.off[r + 1] - off0;
+                const uint64_t off1 = sv.off[r + 1];
+                if (off1 < off0) {  // refused: no op of it or after it runs
+                    S->bad = 1;
+                    S->open = 0;
+                    __hip_atomic_fetch_or(t.status, ST_BAD_INPUT, __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+                const uint64_t sz = off1 - off0;
                 const uint64_t need = sv.reserve ? sv.reserve[r] : sz;
                 if (S->used + need > t.pool && !exact) {  // wait for the exact count
                     S->open = 0;
@@ -536,15 +561,18 @@ __device__ inline void sched_fill(const TableDev& t, const ServeDev& sv, ServeSt
         const bool valid = i < sv.nreq;
         uint64_t sz = 0, need = 0;
         bool arrived = false;
+        bool dec = false;  // decreasing offsets: left to the first-request check
         if (valid) {
-            sz = sv.off[i + 1] - sv.off[i];
+            const uint64_t o0 = sv.off[i], o1 = sv.off[i + 1];
+            dec = o1 < o0;
+            sz = dec ? 0 : o1 - o0;
             need = sv.reserve ? sv.reserve[i] : sz;
             arrived = sv.arrival[i] <= st_el;
         }
         uint64_t tot;
         const uint64_t ps = block_scan(sz, &tot);
         const uint64_t pn = block_scan(need, &tot);
-        const bool fits = valid && S->ops + ps <= sv.max_tick_ops &&
+        const bool fits = valid && !dec && S->ops + ps <= sv.max_tick_ops &&
                           S->n + threadIdx.x + 1 <= (uint64_t)sv.max_tick_reqs &&
                           S->used + S->res + pn <= t.pool;
         const bool ok = fits && arrived;
@@ -594,7 +622,7 @@ __device__ inline void sched_publish(const ServeDev& sv, ServeState* ss) {
         st_coh(&ss->op_hi, op_lo + S->ops);
         ss->next_req = S->r;
         ss->served_ops += S->ops;
-        if (S->n == 0 && S->r >= sv.nreq) st_coh(&ss->done, 1u);
+        if (S->n == 0 && (S->r >= sv.nreq || S->bad)) st_coh(&ss->done, 1u);
         S->bound = S->used + S->res;  // the admitted tick allocates at most its reservations
     }
 }

@@ -145,7 +145,7 @@ int Launch<L>::batches(const LaunchCtx& c, const OpsDev& o, const OutDev& out,
                 // auto mode); of two launches the one that does not match it
                 // returns at once.
                 hipLaunchKernelGGL(k_reset_batches<L>, dim3(1), dim3(1024), 0, s, bar, offsets,
-                                   nbatches, c.run_cap, mode);
+                                   nbatches, c.run_cap, mode, c.dev.status);
                 if (alone)
                     hipExtLaunchKernelGGL((k_batches<L, G, BC, RC, false>), dim3((unsigned)grid),
                                           dim3(PERSIST_THREADS), 0, s, nullptr,

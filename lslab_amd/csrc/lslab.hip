@@ -794,6 +794,7 @@ extern "C" int lslab_status(lslab_table* t, hipStream_t stream) {
     uint32_t st = 0;
     HIPCHK(hipMemcpy(&st, &t->state->status, sizeof(st), hipMemcpyDeviceToHost));
     if (st) HIPCHK(hipMemset(&t->state->status, 0, sizeof(uint32_t)));
+    if (st & ST_BAD_INPUT) return LSLAB_ERR_INVALID;
     if (st & ST_CORRUPT) return LSLAB_ERR_CORRUPT;
     if (st & ST_TIMEOUT) return LSLAB_ERR_TIMEOUT;
     if (st & ST_POOL_EXHAUSTED) return LSLAB_ERR_POOL_EXHAUSTED;

@@ -28,7 +28,9 @@
 namespace lslab_dev {
 
 enum : uint8_t { OP_GET = 0, OP_PUT = 1, OP_REMOVE = 2 };
-enum : uint32_t { ST_POOL_EXHAUSTED = 1u, ST_TIMEOUT = 2u, ST_CORRUPT = 4u };
+// ST_BAD_INPUT: batch or request offsets that decrease (the call did no work
+// past them).
+enum : uint32_t { ST_POOL_EXHAUSTED = 1u, ST_TIMEOUT = 2u, ST_CORRUPT = 4u, ST_BAD_INPUT = 8u };
 
 constexpr uint32_t LOCK_WRITER = 0x80000000u;
 constexpr uint64_t GT_EMPTY = ~0ull;

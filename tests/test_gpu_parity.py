@@ -938,3 +938,31 @@ def test_slot_heads_follow_tuning(lslab, oracle_mod):
     rk, rv = oracle_dump_sorted(ref)
     np.testing.assert_array_equal(k, rk)
     np.testing.assert_array_equal(vv, rv)
+
+
+@pytest.mark.parametrize("runs", [0, 1, 2])
+def test_exec_batches_refuses_decreasing_offsets(lslab, oracle_mod, runs):
+    """Batch offsets that decrease are refused on the device before any op is
+    read: the call leaves the table as it was and the status reads
+    LSLAB_ERR_INVALID; the next valid call runs normally."""
+    say("exec_batches_refuses_decreasing_offsets", runs)
+    kb = vb = 8
+    tbl = lslab.Map(kb, vb, 6, 1 << 10, lslab.tuning(lanes_per_op=2, engine=1,
+                                                     max_subbatch=1 << 10, runs=runs))
+    keys = (np.arange(1, 201, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15))
+    vals = np.arange(1, 201, dtype=np.uint64)
+    ops = np.ones(200, np.uint8)
+    d_ops = torch.from_numpy(ops).cuda()
+    d_keys, d_vals = to_dev(keys, kb), to_dev(vals, vb)
+    bad = torch.tensor([0, 100, 50, 200], dtype=torch.int64, device="cuda")
+    tbl.exec_batches(bad, d_ops, d_keys, d_vals)
+    assert tbl.status() == lslab._lib.ERR_INVALID
+    assert tbl.stats()["present"] == 0
+    good = torch.tensor([0, 100, 150, 200], dtype=torch.int64, device="cuda")
+    f, v = tbl.exec_batches(good, d_ops, d_keys, d_vals)
+    assert tbl.status() == 0
+    assert tbl.stats()["present"] == 200
+    k, vv = dump_sorted(tbl, True, kb, vb)
+    o = np.argsort(keys)
+    np.testing.assert_array_equal(k, keys[o])
+    np.testing.assert_array_equal(vv, vals[o])

@@ -238,3 +238,27 @@ def test_requests_larger_than_a_sub_batch(lslab, oracle_mod, max_blocks):
     o1, o2 = np.argsort(k), np.argsort(rk)
     np.testing.assert_array_equal(k[o1], rk[o2])
     np.testing.assert_array_equal(vv[o1], rvv[o2])
+
+
+@pytest.mark.parametrize("max_blocks", [0, 1])
+def test_refuses_decreasing_request_offsets(lslab, oracle_mod, max_blocks):
+    """Request offsets that decrease end the loop at that request: the
+    requests before it are served (the replay's results), none of its ops or
+    later ones run, and the status reads LSLAB_ERR_INVALID."""
+    say("serve refuses decreasing request offsets", max_blocks)
+    offs = np.array([0, 10, 20, 15, 30])
+    n = 30
+    keys = np.arange(1, n + 1, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15)
+    ops = np.ones(n, np.uint8)
+    vals = keys + np.uint64(3)
+    arrival = np.zeros(4, dtype=np.int64)
+    m = lslab.Map(8, 8, 8, 1 << 10, lslab.tuning(engine=1, max_subbatch=1 << 10,
+                                                 max_blocks=max_blocks))
+    res = m.serve(_t(offs), _t(arrival), _t(ops, np.uint8), _t(keys.view(np.int64)),
+                  _t(vals.view(np.int64)), max_tick_ops=10)
+    assert m.status() == lslab._lib.ERR_INVALID
+    assert res["served_ops"] == 20
+    k, _ = m.dump()
+    assert sorted(k.cpu().numpy().view(np.uint64).tolist()) == sorted(keys[:20].tolist())
+    comp = res["complete_ticks"].cpu().numpy()
+    assert (comp[:2] > 0).all()
