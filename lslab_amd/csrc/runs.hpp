@@ -7,10 +7,10 @@
 //     batch r exchanges on k's entry of region r, which orders the batch's
 //     mutations of k (as the single-batch engine's combining table does);
 //   - key table: per key, the bitmap of the run's batches that mutate it.
-// Pass A runs the mutations only: wavefront combining by (key, batch), one
-// exchange per wavefront chain on the chain table; a mutation with a
-// predecessor in its chain gets its result at once, and the first mutation of
-// each (key, batch) chain sets the batch's bit in the key's bitmap.  After a
+// Pass A runs the mutations only: one exchange per mutation on its (key,
+// batch) entry of the chain table; a mutation with a predecessor gets its
+// result at once, and the first mutation of each (key, batch) sets the
+// batch's bit in the key's bitmap.  After a
 // grid barrier the tables are read-only, so pass B reads them with plain,
 // cacheable loads: every get of batch r, and every first mutation of a chain
 // of batch r, takes the key's state from the last mutation of the nearest
@@ -75,8 +75,10 @@ __device__ __forceinline__ void run_region(const RunOffs& ro, int r, uint64_t& b
     cmask = (uint32_t)(cap - 1);
 }
 
-// Claims or finds key's entry in region r (one compare-and-swap per probe);
-// won: this call claimed it (exactly one call per (key, batch) does).  The
+// Claims or finds key's entry in region r: a coherent load of each probed
+// slot, and a compare-and-swap only on an empty one (the mutations of a hot
+// key in one batch find its entry without a read-modify-write); won: this
+// call claimed it (exactly one call per (key, batch) does).  The
 // all-ones key has a dedicated entry per batch after the regions, whose
 // claim is the exchange that finds its head empty.
 __device__ __forceinline__ uint32_t rc_claim(const TableDev& t, const RunTabs& rt, uint64_t key, uint64_t base,
@@ -85,13 +87,17 @@ __device__ __forceinline__ uint32_t rc_claim(const TableDev& t, const RunTabs& r
     if (key == GT_EMPTY) return (uint32_t)(t.rc_cap + r);
     uint32_t h = (uint32_t)mix64(key) & cmask;
     for (uint32_t p = 0; p <= cmask; ++p) {
-        uint64_t e = GT_EMPTY;
-        if (__hip_atomic_compare_exchange_strong(&rt.rc[base + h].key, &e, key, __ATOMIC_RELAXED,
-                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-            won = true;
-            return (uint32_t)(base + h);
+        const uint64_t cur = ld_agent(&rt.rc[base + h].key);
+        if (cur == key) return (uint32_t)(base + h);
+        if (cur == GT_EMPTY) {
+            uint64_t e = GT_EMPTY;
+            if (__hip_atomic_compare_exchange_strong(&rt.rc[base + h].key, &e, key, __ATOMIC_RELAXED,
+                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                won = true;
+                return (uint32_t)(base + h);
+            }
+            if (e == key) return (uint32_t)(base + h);
         }
-        if (e == key) return (uint32_t)(base + h);
         h = (h + 1u) & cmask;
     }
     return NOENT;  // unreachable: a region is at most half full
@@ -141,11 +147,18 @@ __device__ __forceinline__ uint32_t rk_claim(const TableDev& t, const RunTabs& r
     if (key == GT_EMPTY) return t.rk_mask + 1u;
     uint32_t h = (uint32_t)mix64(key) & t.rk_mask;
     for (uint32_t p = 0; p <= t.rk_mask; ++p) {
-        uint64_t e = GT_EMPTY;
-        if (__hip_atomic_compare_exchange_strong(&rt.rk[h].key, &e, key, __ATOMIC_RELAXED,
-                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
-            e == key)
-            return h;
+        // Keys are only ever added during a run: a coherent load that finds
+        // the key skips the compare-and-swap (a Zipf-hot key is registered
+        // by its first chain in every batch of the run).
+        const uint64_t cur = ld_agent(&rt.rk[h].key);
+        if (cur == key) return h;
+        if (cur == GT_EMPTY) {
+            uint64_t e = GT_EMPTY;
+            if (__hip_atomic_compare_exchange_strong(&rt.rk[h].key, &e, key, __ATOMIC_RELAXED,
+                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
+                e == key)
+                return h;
+        }
         h = (h + 1u) & t.rk_mask;
     }
     return NOENT;
@@ -225,14 +238,15 @@ __device__ __forceinline__ uint32_t* run_a_buf() {
 }
 
 // One round of pass A: the wavefront's c <= 64 gathered mutations, one per
-// lane.  Mutations of one (key, batch) are chained in lane order; the chain's
-// last op exchanges on the (key, batch) entry of the chain table.  A mutation
-// whose predecessor is known (an earlier op of its chain, or the chain that
-// exchanged before) writes its result now; the first mutation of a (key,
-// batch) chain is left to pass B (run_pred = RUN_CF | entry).  The op that
-// claims a (key, batch) entry registers the batch in the key's bitmap and
-// lists the entry for clearing (clr, counted by ccnt in LDS), in parallel
-// with its exchange.
+// lane.  Each mutation exchanges itself on its (key, batch) entry of the
+// chain table, so the exchanges order the batch's mutations of the key.  A
+// mutation whose exchange returns a predecessor writes its result now; the
+// first of a (key, batch) is left to pass B (run_pred = RUN_CF | entry).  The
+// op that claims a (key, batch) entry registers the batch in the key's bitmap
+// and lists the entry for clearing (clr, counted by ccnt in LDS), in parallel
+// with its exchange.  (Chaining a wavefront's mutations of one key in lane
+// order first, one exchange per chain, cost more in the ballot loop that
+// finds the chains than the exchanges it saved on this workload.)
 template <class L>
 __device__ __forceinline__ void run_a_round(const TableDev& t, const RunTabs& rt, const OpsDev& o,
                                             const OutDev& out, uint64_t lo, uint64_t m,
@@ -254,26 +268,9 @@ __device__ __forceinline__ void run_a_round(const TableDev& t, const RunTabs& rt
         r = run_batch_of(ro, li);
     }
     const uint64_t kk = (uint64_t)key;
-    uint64_t todo = __ballot(lead);
-    int my_last = lane, my_prev = -1;
-    while (todo) {
-        const int l = __builtin_ctzll(todo);
-        const uint64_t kl = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(kk >> 32), l) << 32) |
-                            (uint32_t)__builtin_amdgcn_readlane((uint32_t)kk, l);
-        const int rl = __builtin_amdgcn_readlane(r, l);
-        const bool mine = lead && kk == kl && r == rl;
-        const uint64_t same = __ballot(mine);
-        todo &= ~same;
-        if (mine) {
-            my_last = 63 - __builtin_clzll(same);
-            const uint64_t below = same & ((1ull << lane) - 1ull);
-            my_prev = below ? 63 - __builtin_clzll(below) : -1;
-        }
-    }
-    const bool rep = lead && lane == my_last;
     uint32_t ce = NOENT, fpred = 0;
     bool won = false;
-    if (rep) {
+    if (lead) {
         uint64_t base;
         uint32_t cmask;
         run_region(ro, r, base, cmask);
@@ -294,16 +291,12 @@ __device__ __forceinline__ void run_a_round(const TableDev& t, const RunTabs& rt
             corrupt(t);
         }
     }
-    const uint32_t fp = __shfl(fpred, my_last, 64);
-    const uint32_t cc = __shfl(ce, my_last, 64);
-    const uint32_t prev_li = __shfl(li, my_prev >= 0 ? my_prev : lane, 64);
     if (lead) {
-        const uint32_t pr = my_prev >= 0 ? prev_li + 1u : fp;
-        if (pr != 0) {
-            st_coh(&t.run_pred[li], pr);
-            mut_result<L>(t, o, out, lo, m, li, op, pr, false, V(0));
+        if (fpred != 0) {
+            st_coh(&t.run_pred[li], fpred);
+            mut_result<L>(t, o, out, lo, m, li, op, fpred, false, V(0));
         } else {
-            st_coh(&t.run_pred[li], RUN_CF | cc);
+            st_coh(&t.run_pred[li], RUN_CF | ce);
         }
     }
     const uint64_t fm = __ballot(won);
