@@ -392,7 +392,6 @@ __device__ __forceinline__ void run_b_op(const TableDev& t, const RunTabs& rt, c
     const uint64_t kk = (uint64_t)key;
     const uint64_t b = bucket_of(t, kk);
     K blk[S];
-    probe_issue<L, G, false>(t, b, blk);
     uint4 w0 = make_uint4(0, 0, 0, 0);
     uint64_t w1 = 0;
     if (gl == 0) rk_first(t, rt, kk, w0, w1);
@@ -411,6 +410,7 @@ __device__ __forceinline__ void run_b_op(const TableDev& t, const RunTabs& rt, c
     int slot = -1;
     if (!earlier || last) {
         uint32_t valid;
+        probe_issue<L, G, false>(t, b, blk);
         walk_on<L, G, false>(t, probe_compute<L, G, false>(t, b, key, blk), s, slot, valid, key);
         if constexpr (L::IS_MAP) {
             if (slot >= 0 && !earlier && gl == 0) cur = *val_slot<L>(t, s, slot);
@@ -454,11 +454,13 @@ __device__ __forceinline__ void run_b_op(const TableDev& t, const RunTabs& rt, c
 }
 
 // Pass B over the run's ops: every get and every first mutation of a (key,
-// batch) chain.  The table probe and the key table's first slot are issued
-// together (both depend on the key alone); the probe finishes only when the
-// key has no earlier mutation in the run or this is the key's last chain (it
-// then gives the key's location for phase W).  A record's opcode, key, value
-// and run_pred are loaded together, one op ahead.
+// batch) chain.  The key table is read first; the table is probed only when
+// the key has no earlier mutation in the run (the op reads the table's
+// value) or this is the key's last chain (the probe gives the key's location
+// for phase W).  Under skew most ops of a hot key need neither, so probing
+// after the key-table lookup, not beside it, saves more bucket-line reads
+// than the serialisation costs.  A record's opcode, key and run_pred are
+// loaded together, one op ahead.
 template <class L, int G>
 __device__ __forceinline__ void run_pass_b(const TableDev& t, const RunTabs& rt, const OpsDev& o,
                                            const OutDev& out, uint64_t lo, uint64_t m,
