@@ -278,14 +278,23 @@ __device__ __forceinline__ void run_a_round(const TableDev& t, const RunTabs& rt
         if (ce != NOENT) {
             fpred = __hip_atomic_exchange(&rt.rc[ce].head, li + 1u, __ATOMIC_RELAXED,
                                           __HIP_MEMORY_SCOPE_AGENT);
-            if (kk == GT_EMPTY) won = fpred == 0;  // its dedicated entry's first exchange
-            if (won) {
+            auto enroll = [&]() {  // the batch into the key's bitmap
                 const uint32_t gk = rk_claim(t, rt, kk);
                 if (gk != NOENT)
                     __hip_atomic_fetch_or(&rt.rk[gk].mask[r >> 6], 1ull << (r & 63),
                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 else
                     corrupt(t);
+            };
+            // The claim alone says whether an ordinary key's op registers
+            // the batch, so its registration is issued while the exchange is
+            // in flight.  The reserved key has a dedicated entry per batch
+            // and no claim: its first exchange registers.
+            if (kk != GT_EMPTY) {
+                if (won) enroll();
+            } else {
+                won = fpred == 0;
+                if (won) enroll();
             }
         } else {
             corrupt(t);
