@@ -519,19 +519,16 @@ __device__ __forceinline__ void run_phase_w(const TableDev& t, const RunTabs& rt
         const uint8_t op = ld_opcode(o, i);
         const K key = ld_stride<K>(o.keys, o.kstride, i);
         const V val = (L::IS_MAP && o.vals) ? ld_stride<V>(o.vals, o.vstride, i) : V(0);
-        if (gl == 0) {
-            st_coh(&rt.rk[e.g].key, GT_EMPTY);
-            st_coh(&rt.rk[e.g].mask[0], (uint64_t)0);
-            st_coh(&rt.rk[e.g].mask[1], (uint64_t)0);
+        if (gl == 0) {  // the 32-B entry: key and mask[0], then mask[1] and pad
+            st_coh16(&rt.rk[e.g].key, GT_EMPTY, 0);
+            st_coh16(&rt.rk[e.g].mask[1], 0, 0);
         }
         w_apply<L, G, RC>(t, op, key, val, e.loc);
     }
     for (uint32_t k = threadIdx.x; k < ccnt; k += blockDim.x) {
         const uint32_t c = ld_coh(&clr[k]);
-        if (c < t.rc_cap + RUN_MAX_BATCHES) {
-            st_coh(&rt.rc[c].key, GT_EMPTY);
-            st_coh(&rt.rc[c].head, 0u);
-        }
+        if (c < t.rc_cap + RUN_MAX_BATCHES)  // the 16-B entry: key, head and pad
+            st_coh16(&rt.rc[c], GT_EMPTY, 0);
     }
 }
 

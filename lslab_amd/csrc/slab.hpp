@@ -90,6 +90,10 @@ struct RunChain {
     uint32_t head;  // last exchanged mutation (run op index + 1)
     uint32_t pad;
 };
+// Phase W clears an entry with 16-B stores: key table as (key, mask[0]) and
+// (mask[1], pad), chain table as (key, head | pad).
+static_assert(sizeof(RunKey) == 32 && offsetof(RunKey, mask) == 8, "RunKey layout");
+static_assert(sizeof(RunChain) == 16 && offsetof(RunChain, head) == 8, "RunChain layout");
 
 // One set of run tables (the batched kernel alternates two, so that one
 // run's write-back overlaps the next run's first pass).
@@ -250,6 +254,15 @@ __device__ __forceinline__ T ld_coh(const T* p) {
 template <typename T>
 __device__ __forceinline__ void st_coh(T* p, T v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Write-through store of 16 bytes (lo at p, hi at p + 8): the encoding of an
+// agent-scope store (sc1), four words wide.  For clearing a run-table entry
+// in one store where each word only has to be visible after the next grid
+// barrier (which waits for it), not as one atomic unit.  p: 8-B aligned.
+__device__ __forceinline__ void st_coh16(void* p, uint64_t lo, uint64_t hi) {
+    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4 w = {(unsigned)lo, (unsigned)(lo >> 32), (unsigned)hi, (unsigned)(hi >> 32)};
+    __asm__ volatile("global_store_dwordx4 %0, %1, off sc1" ::"v"(p), "v"(w) : "memory");
 }
 // Waits until this wave's earlier memory operations have completed; with
 // every shared write issued as an agent-scope store or atomic, this is all a
